@@ -1,0 +1,170 @@
+/*
+ * ns3d.h -- C ABI of the MI355X (gfx950) NS3DPeriodic time-stepping hot path.
+ *
+ * Plain C: opaque context, raw device pointers, sizes and scalars; every call
+ * returns an int status (NS3D_OK = 0) except the Krylov callbacks, whose
+ * signature is fixed by the reference (void(*)(void*,void*,void*)).
+ *
+ * Data layout is the reference's (Source/Macros.h:11,15):
+ *   physical field  double [Nx][Ny][Nz]
+ *   spectral field  ns3d_cplx [Nx][Ny][Mz], Mz = Nz/2+1, z fastest,
+ *                   unnormalised forward DFT (sign -1).
+ * ns3d_cplx has the layout of cufftDoubleComplex / hipDoubleComplex.
+ * All array arguments are device pointers on the context's device unless
+ * stated.  Calls are asynchronous on the context's stream unless they return
+ * a host value.
+ *
+ * Reference entry points replaced (paths relative to the reference root):
+ *   RK_time_step.h:17-34, advection_2_3.h:34-46, advection_WENO.h:19-23,
+ *   diffusion.h:12-24, divergence.h:13-15, min_max_reduction.h:33-37,
+ *   math_support.h:15-72, Arnoldi/Matrix_Vector_emulator.h:11.
+ */
+#ifndef NS3D_H
+#define NS3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ns3d_ctx ns3d_ctx;
+typedef struct { double x, y; } ns3d_cplx;
+
+/* status codes */
+#define NS3D_OK 0
+#define NS3D_ERR_ARG (-1)          /* null pointer / bad value */
+#define NS3D_ERR_SIZE (-2)         /* grid size without a compiled FFT plan */
+#define NS3D_ERR_ALLOC (-3)        /* device allocation failed */
+#define NS3D_ERR_HIP (-4)          /* HIP runtime error (launch or copy) */
+#define NS3D_ERR_NONFINITE (-5)    /* NaN/Inf detected by a checking call */
+#define NS3D_ERR_PRECISION (-6)    /* operation not offered at this precision */
+
+/* precision of the transform / product bulk; state arrays are always FP64 */
+#define NS3D_FP64 0
+#define NS3D_FP32_BULK 1
+
+/* behaviour flags (reference quirks are ON in reference mode) */
+#define NS3D_FLAG_NO_IPQ 0x1u      /* disable zeroing Re(Q^) and Q^(0) (advection_2_3.cu:35-70) */
+#define NS3D_FLAG_WENO_ZFIX 0x2u   /* use l-2 in flux_z_weno_R (advection_WENO.cu:288 reads l-1) */
+
+typedef struct {
+    int Nx, Ny, Nz;        /* physical grid, each in {8,16,32,48,64,96,128,192,256,512} */
+    double Lx, Ly, Lz;     /* box lengths (reference: 2*pi/alpha, 2*pi, 2*pi/beta, NS3D_periodic.cu:969-979) */
+    double Re;             /* Reynolds number, nu = 1/Re */
+    double dt;             /* time step */
+    int precision;         /* NS3D_FP64 or NS3D_FP32_BULK */
+    unsigned flags;        /* NS3D_FLAG_* */
+    void *stream;          /* hipStream_t; NULL = default stream */
+    int device;            /* HIP device ordinal; -1 = current device */
+} ns3d_config;
+
+/* ---- context (replaces struct_NS3D_RK3_Call, Structures.h:6-58, and
+ *      init_dealiasing/clean_dealiasing, advection_2_3.cu:529-585) ---- */
+int ns3d_create(const ns3d_config *cfg, ns3d_ctx **out);
+int ns3d_destroy(ns3d_ctx *ctx);
+int ns3d_set_params(ns3d_ctx *ctx, double dt, double Re);
+int ns3d_set_stream(ns3d_ctx *ctx, void *stream);
+int ns3d_synchronize(ns3d_ctx *ctx);
+int ns3d_last_error(const ns3d_ctx *ctx);          /* status of the last void-returning callback */
+const char *ns3d_status_string(int status);
+int ns3d_supported_size(int n);                    /* 1 if an FFT plan for length n is compiled */
+long long ns3d_workspace_bytes(const ns3d_ctx *ctx);
+
+/* ---- a4: FFT layer (math_support.cu:301-345) ---- */
+int ns3d_fft_r2c(ns3d_ctx *ctx, const double *in, ns3d_cplx *out);              /* FFTN_Device */
+int ns3d_fft_c2r(ns3d_ctx *ctx, const ns3d_cplx *in, double *out);              /* iFFTN_Device(dimGrid,..) incl. scale_double */
+int ns3d_fft_c2r_unscaled(ns3d_ctx *ctx, const ns3d_cplx *in, double *out);     /* iFFTN_Device(source,dest) */
+
+/* ---- a5: spectral tables (math_support.cu:374-519).  The kernels recompute
+ *      them on the fly; this writes the reference arrays for callers that
+ *      need them.  Any pointer may be NULL.  kx:[Nx] ky:[Ny] kz:[Mz], others [Nx][Ny][Mz]. ---- */
+int ns3d_tables(ns3d_ctx *ctx, double *kx, double *ky, double *kz, double *din_diffusion, double *din_poisson,
+                double *AM11, double *AM22, double *AM33, double *AM12, double *AM13, double *AM23, double *mask);
+
+/* ---- a3: nonlinear terms ---- */
+int ns3d_nonlinear(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
+                   ns3d_cplx *Qx, ns3d_cplx *Qy, ns3d_cplx *Qz);                  /* calculate_convolution_2p3 */
+int ns3d_bilinear(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
+                  const ns3d_cplx *vx, const ns3d_cplx *vy, const ns3d_cplx *vz,
+                  ns3d_cplx *Qx, ns3d_cplx *Qy, ns3d_cplx *Qz);                   /* calculate_convolution_2p3_UV */
+
+/* ---- a6: projection / diffusion / update family (diffusion.cu:96-416) ---- */
+#define NS3D_UPD_STAGE 0    /* solve_advection_diffusion_projection:      u <- u + dt(AM Q - Q - |k|^2/Re u + f) */
+#define NS3D_UPD_UV_RHS 1   /* solve_advection_diffusion_projection_UV_RHS: u <- AM Q - Q - |k|^2/Re u */
+#define NS3D_UPD_UV 2       /* solve_advection_diffusion_projection_UV:     u <- u + dt(AM Q - Q - |k|^2/Re u) */
+#define NS3D_UPD_IUV 3      /* solve_advection_implicit_diffusion_projection_UV: u <- (u + dt(AM Q - Q))/(1+dt|k|^2/Re) */
+#define NS3D_UPD_RHS 4      /* RHS_advection_diffusion_projection:          R <- AM Q - Q - |k|^2/Re u + f */
+int ns3d_project_update(ns3d_ctx *ctx, int mode, ns3d_cplx *ux, ns3d_cplx *uy, ns3d_cplx *uz,
+                        const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz,
+                        const ns3d_cplx *Qx, const ns3d_cplx *Qy, const ns3d_cplx *Qz,
+                        ns3d_cplx *Rx, ns3d_cplx *Ry, ns3d_cplx *Rz);
+
+/* ---- a7: stage combination kernels (RK_time_step.cu:6-39,82-115) ---- */
+int ns3d_copy3(ns3d_ctx *ctx, const ns3d_cplx *sx, const ns3d_cplx *sy, const ns3d_cplx *sz,
+               ns3d_cplx *dx, ns3d_cplx *dy, ns3d_cplx *dz);                      /* copy_arrays */
+int ns3d_axpby3(ns3d_ctx *ctx, const ns3d_cplx *ax, const ns3d_cplx *ay, const ns3d_cplx *az, double wa,
+                const ns3d_cplx *bx, const ns3d_cplx *by, const ns3d_cplx *bz, double wb,
+                ns3d_cplx *dx, ns3d_cplx *dy, ns3d_cplx *dz);                     /* single_RK_step */
+
+/* ---- a2/a1: stages and the SSP-RK3 step (RK_time_step.cu:231-327).  Forcing
+ *      pointers may all be NULL (zero forcing). ---- */
+int ns3d_stage(ns3d_ctx *ctx, ns3d_cplx *ux, ns3d_cplx *uy, ns3d_cplx *uz,
+               const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz);    /* single_time_step */
+int ns3d_rhs(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
+             const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz,
+             ns3d_cplx *Rx, ns3d_cplx *Ry, ns3d_cplx *Rz);                        /* return_RHS */
+int ns3d_rk3_ssp(ns3d_ctx *ctx, ns3d_cplx *ux, ns3d_cplx *uy, ns3d_cplx *uz,
+                 const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz);  /* RK3_SSP */
+int ns3d_rk3_ssp_steps(ns3d_ctx *ctx, int nsteps, ns3d_cplx *ux, ns3d_cplx *uy, ns3d_cplx *uz,
+                       const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz);
+
+/* ---- a8: divergence (divergence.cu:8-47) ---- */
+int ns3d_divergence(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
+                    ns3d_cplx *div_hat);                                          /* divergence_device */
+int ns3d_divergence_physical(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
+                             ns3d_cplx *div_hat, double *div);                    /* + devergence_to_double */
+
+/* ---- a9: WENO advection (advection_WENO.cu:304-415).  scheme 3 or 5;
+ *      any other value gives the first-order upwind value, as in the reference. ---- */
+int ns3d_weno_advect(ns3d_ctx *ctx, int scheme, const double *ux, const double *uy, const double *uz,
+                     const double *x, double *out);                               /* advection kernel, scalar x */
+int ns3d_weno_advection(ns3d_ctx *ctx, int scheme, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
+                        ns3d_cplx *Qx, ns3d_cplx *Qy, ns3d_cplx *Qz);             /* WenoAdvection */
+
+/* ---- a10: J.v and the tangent map (RK_time_step.cu:266-355, NS3D_periodic.cu:95-470) ---- */
+int ns3d_rk3_ssp_uv_rhs(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
+                        ns3d_cplx *vx, ns3d_cplx *vy, ns3d_cplx *vz);             /* RK3_SSP_UV_RHS */
+int ns3d_stage_uv(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
+                  ns3d_cplx *vx, ns3d_cplx *vy, ns3d_cplx *vz);                   /* single_time_step_UV */
+int ns3d_stage_iuv_rhs(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
+                       ns3d_cplx *vx, ns3d_cplx *vy, ns3d_cplx *vz);              /* single_time_step_iUV_RHS */
+int ns3d_rk3_ssp_uv(ns3d_ctx *ctx, ns3d_cplx *ux, ns3d_cplx *uy, ns3d_cplx *uz,
+                    ns3d_cplx *vx, ns3d_cplx *vy, ns3d_cplx *vz,
+                    const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz); /* RK3_SSP_UV (advances u too) */
+int ns3d_unpack_reduced(ns3d_ctx *ctx, const double *vec, ns3d_cplx *vx, ns3d_cplx *vy, ns3d_cplx *vz);
+int ns3d_pack_reduced(ns3d_ctx *ctx, const ns3d_cplx *vx, const ns3d_cplx *vy, const ns3d_cplx *vz, double *vec);
+int ns3d_unpack_full(ns3d_ctx *ctx, const double *vec, ns3d_cplx *vx, ns3d_cplx *vy, ns3d_cplx *vz);
+int ns3d_pack_full(ns3d_ctx *ctx, const ns3d_cplx *vx, const ns3d_cplx *vy, const ns3d_cplx *vz, double *vec);
+/* Base flow, forcing and period used by the callbacks below (the fields of
+ * struct_NS3D_RK3_Call the reference callbacks read). */
+int ns3d_set_base_state(ns3d_ctx *ctx, ns3d_cplx *ux, ns3d_cplx *uy, ns3d_cplx *uz,
+                        const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz,
+                        int timesteps_period, double tau);
+/* user_map_vector callbacks (Arnoldi/Matrix_Vector_emulator.h:11); vectors are device double*. */
+void ns3d_jv_reduced(void *ctx, void *vec_in, void *vec_out);                  /* NSCallMatrixVector_reduced */
+void ns3d_jv_reduced_newton(void *ctx, void *vec_in, void *vec_out);           /* ..._reduced_Newton_Globalization */
+void ns3d_jv_exponential(void *ctx, void *vec_in, void *vec_out);              /* NSCallMatrixVector_exponential */
+
+/* ---- a11/a12: reductions and the diagnostics that feed them.  Results are
+ *      returned on the host (synchronous, like the reference). ---- */
+int ns3d_reduce_minmax(ns3d_ctx *ctx, long long n, const double *in, double *out2_host);  /* compute_reduction: [min,max] */
+int ns3d_reduce_sum(ns3d_ctx *ctx, long long n, const double *in, double *out_host);      /* reduction_sum */
+int ns3d_kinetic_energy(ns3d_ctx *ctx, const double *ux, const double *uy, const double *uz, double *out_host);
+int ns3d_dissipation(ns3d_ctx *ctx, const double *ux, const double *uy, const double *uz, double *out_host);
+int ns3d_cfl_dt(ns3d_ctx *ctx, double cfl, const double *ux, const double *uy, const double *uz, double *dt_host);
+int ns3d_shapiro_error(ns3d_ctx *ctx, double t, const double *ux, const double *uy, const double *uz,
+                       double *err_host);                                          /* Shapiro_test_case: L-inf error */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,252 @@
+// fft_core.hpp -- register/LDS mixed-radix Stockham FFT building blocks for gfx950.
+//
+// A "pencil" is one 1D transform of length N.  A workgroup transforms P
+// pencils at once; TPP threads cooperate on each pencil.  Every pass with
+// radix R performs N/R butterflies; thread t owns butterflies t, t+TPP, ...
+// Data live in registers; between passes they are exchanged through LDS.
+//
+// Pass i (Stockham autosort, Ns = product of the radices before i):
+//   inputs  x[jj + r*N/R],  r < R
+//   twiddle x_r *= W^(r*(jj mod Ns)*N/(Ns*R)),  W = exp(DIR*2*pi*i/N)
+//   R-point DFT
+//   outputs y[(jj/Ns)*Ns*R + (jj mod Ns) + r*Ns]
+// so the first pass reads x[jj + r*N/R1] and the last pass writes
+// y[jj + r*N/Rlast]: callers load and store those patterns directly from and
+// to global memory, and only the inner exchanges touch LDS.
+#pragma once
+#include "ns3d_common.hpp"
+
+namespace ns3d {
+
+// ---------------------------------------------------------------- radix DFTs
+template <int R> struct Radix;
+
+template <> struct Radix<2> {
+    template <int DIR, typename T> __device__ __forceinline__ static void dft(cplx<T>* v) {
+        cplx<T> a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    }
+};
+
+template <> struct Radix<3> {
+    template <int DIR, typename T> __device__ __forceinline__ static void dft(cplx<T>* v) {
+        const T h = (T)0.86602540378443864676372317075294;  // sqrt(3)/2
+        cplx<T> a0 = v[0], a1 = v[1], a2 = v[2];
+        cplx<T> s = a1 + a2;
+        cplx<T> d = a1 - a2;
+        cplx<T> m = mk<T>(a0.x - (T)0.5 * s.x, a0.y - (T)0.5 * s.y);
+        cplx<T> r = mul_i<DIR>(cscale(d, h));
+        v[0] = a0 + s;
+        v[1] = m + r;
+        v[2] = m - r;
+    }
+};
+
+template <> struct Radix<4> {
+    template <int DIR, typename T> __device__ __forceinline__ static void dft(cplx<T>* v) {
+        cplx<T> s02 = v[0] + v[2], d02 = v[0] - v[2];
+        cplx<T> s13 = v[1] + v[3], d13 = mul_i<DIR>(v[1] - v[3]);
+        v[0] = s02 + s13;
+        v[1] = d02 + d13;
+        v[2] = s02 - s13;
+        v[3] = d02 - d13;
+    }
+};
+
+template <> struct Radix<8> {
+    template <int DIR, typename T> __device__ __forceinline__ static void dft(cplx<T>* v) {
+        const T c = (T)0.70710678118654752440084436210485;  // 1/sqrt(2)
+        cplx<T> e[4] = {v[0], v[2], v[4], v[6]};
+        cplx<T> o[4] = {v[1], v[3], v[5], v[7]};
+        Radix<4>::dft<DIR>(e);
+        Radix<4>::dft<DIR>(o);
+        // W8^1 = (1 + DIR i)/sqrt2, W8^2 = DIR i, W8^3 = (-1 + DIR i)/sqrt2
+        cplx<T> o1 = cscale(o[1] + mul_i<DIR>(o[1]), c);
+        cplx<T> o2 = mul_i<DIR>(o[2]);
+        cplx<T> o3 = cscale(mul_i<DIR>(o[3]) - o[3], c);
+        v[0] = e[0] + o[0];
+        v[4] = e[0] - o[0];
+        v[1] = e[1] + o1;
+        v[5] = e[1] - o1;
+        v[2] = e[2] + o2;
+        v[6] = e[2] - o2;
+        v[3] = e[3] + o3;
+        v[7] = e[3] - o3;
+    }
+};
+
+template <> struct Radix<16> {
+    template <int DIR, typename T> __device__ __forceinline__ static void dft(cplx<T>* v) {
+        const T c1 = (T)0.92387953251128675612818318939679;  // cos(pi/8)
+        const T s1 = (T)0.38268343236508977172845998403040;  // sin(pi/8)
+        const T c2 = (T)0.70710678118654752440084436210485;
+        // columns n2: y[n2][k1] = DFT4 over n1 of v[4*n1+n2]
+        cplx<T> y[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; n2++) {
+            cplx<T> a[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+            Radix<4>::dft<DIR>(a);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; k1++) y[n2][k1] = a[k1];
+        }
+        // twiddles W16^(n2*k1), W16 = exp(DIR*i*pi/8)
+        const cplx<T> w1 = mk<T>(c1, (T)DIR * s1), w2 = mk<T>(c2, (T)DIR * c2), w3 = mk<T>(s1, (T)DIR * c1);
+        const cplx<T> w6 = mk<T>(-c2, (T)DIR * c2), w9 = mk<T>(-c1, (T)(-DIR) * s1);
+        y[1][1] = cmul(y[1][1], w1);
+        y[1][2] = cmul(y[1][2], w2);
+        y[1][3] = cmul(y[1][3], w3);
+        y[2][1] = cmul(y[2][1], w2);
+        y[2][2] = mul_i<DIR>(y[2][2]);
+        y[2][3] = cmul(y[2][3], w6);
+        y[3][1] = cmul(y[3][1], w3);
+        y[3][2] = cmul(y[3][2], w6);
+        y[3][3] = cmul(y[3][3], w9);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            cplx<T> a[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+            Radix<4>::dft<DIR>(a);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = a[k2];
+        }
+    }
+};
+
+// ---------------------------------------------------------------- plans
+template <int... R> struct Seq {};
+
+template <typename S> struct SeqInfo;
+template <int... R> struct SeqInfo<Seq<R...>> {
+    static constexpr int n = sizeof...(R);
+    static constexpr int rad[n] = {R...};
+    static constexpr int prod_before(int i) {
+        int p = 1;
+        for (int m = 0; m < i; m++) p *= rad[m];
+        return p;
+    }
+};
+
+template <typename S> struct SeqRev;
+template <int... R> struct SeqRevAcc;
+template <> struct SeqRev<Seq<>> { using type = Seq<>; };
+template <int A, int... R> struct SeqRev<Seq<A, R...>> {
+    template <typename S2> struct Append;
+    template <int... B> struct Append<Seq<B...>> { using type = Seq<B..., A>; };
+    using type = typename Append<typename SeqRev<Seq<R...>>::type>::type;
+};
+
+// Plans: radices with at most 8 elements per thread per pass (TPP = N/8 for
+// the powers of two).  The inverse plan of the fused z-pass and the forward
+// plan are mirror images so the product stage needs no exchange.
+template <int N> struct Plan;
+template <> struct Plan<8> { using R = Seq<8>; static constexpr int TPP = 1; };
+template <> struct Plan<16> { using R = Seq<2, 8>; static constexpr int TPP = 2; };
+template <> struct Plan<24> { using R = Seq<3, 8>; static constexpr int TPP = 3; };
+template <> struct Plan<32> { using R = Seq<4, 8>; static constexpr int TPP = 4; };
+template <> struct Plan<48> { using R = Seq<3, 2, 8>; static constexpr int TPP = 6; };
+template <> struct Plan<64> { using R = Seq<8, 8>; static constexpr int TPP = 8; };
+template <> struct Plan<96> { using R = Seq<3, 4, 8>; static constexpr int TPP = 12; };
+template <> struct Plan<128> { using R = Seq<2, 8, 8>; static constexpr int TPP = 16; };
+template <> struct Plan<192> { using R = Seq<3, 8, 8>; static constexpr int TPP = 24; };
+template <> struct Plan<256> { using R = Seq<4, 8, 8>; static constexpr int TPP = 32; };
+template <> struct Plan<384> { using R = Seq<3, 2, 8, 8>; static constexpr int TPP = 48; };
+template <> struct Plan<512> { using R = Seq<8, 8, 8>; static constexpr int TPP = 64; };
+
+// Registers per thread for a plan: max over passes of ceil((N/R)/TPP)*R.
+template <int N, int TPP, typename S> struct PlanRegs;
+template <int N, int TPP, int... R> struct PlanRegs<N, TPP, Seq<R...>> {
+    static constexpr int one(int r) { return ((N / r + TPP - 1) / TPP) * r; }
+    static constexpr int value() {
+        int e = 0;
+        const int rr[] = {R...};
+        for (int i = 0; i < (int)sizeof...(R); i++) e = one(rr[i]) > e ? one(rr[i]) : e;
+        return e;
+    }
+};
+
+// ------------------------------------------------------------- pencil FFT
+// LDS addressing policies.  IndexMajor: element idx of pencil p at idx*P+p
+// (pencils are consecutive z-columns: conflict-free when lanes are p-fastest).
+// PencilMajor: p*(N+PAD)+idx (pencils contiguous; lanes t-fastest).
+template <int P> struct IndexMajor {
+    __device__ __forceinline__ static int at(int p, int idx) { return idx * P + p; }
+};
+template <int N, int PAD> struct PencilMajor {
+    __device__ __forceinline__ static int at(int p, int idx) { return p * (N + PAD) + idx; }
+};
+
+template <typename T, int N, int TPP, int DIR, typename S, typename Addr>
+struct PencilFFT {
+    using Info = SeqInfo<S>;
+    static constexpr int NP = Info::n;
+    static constexpr int E = PlanRegs<N, TPP, S>::value();
+    static constexpr int R0 = Info::rad[0];
+    static constexpr int RL = Info::rad[NP - 1];
+    // first-pass input pattern: v[q*R0 + r] <-> x[t + q*TPP + r*N/R0]
+    static constexpr int B0 = (N / R0 + TPP - 1) / TPP;
+    // last-pass output pattern: v[q*RL + r] <-> y[t + q*TPP + r*N/RL]
+    static constexpr int BL = (N / RL + TPP - 1) / TPP;
+
+    __device__ __forceinline__ static bool in_active(int t, int q) { return (N / R0) % TPP == 0 || t + q * TPP < N / R0; }
+    __device__ __forceinline__ static int in_index(int t, int q, int r) { return t + q * TPP + r * (N / R0); }
+    __device__ __forceinline__ static bool out_active(int t, int q) { return (N / RL) % TPP == 0 || t + q * TPP < N / RL; }
+    __device__ __forceinline__ static int out_index(int t, int q, int r) { return t + q * TPP + r * (N / RL); }
+
+    // tw[m] = exp(-2*pi*i*m/N); DIR=+1 uses the conjugate.
+    template <int I>
+    __device__ __forceinline__ static void pass(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t, int p) {
+        constexpr int R = Info::rad[I];
+        constexpr int Ns = Info::prod_before(I);
+        constexpr int NB = N / R;
+        constexpr int B = (NB + TPP - 1) / TPP;
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int jj = t + q * TPP;
+            if (NB % TPP == 0 || jj < NB) {
+                if constexpr (Ns > 1) {
+                    const int jm = jj % Ns;
+#pragma unroll
+                    for (int r = 1; r < R; r++) {
+                        cplx<T> w = tw[r * jm * (N / (Ns * R))];
+                        if (DIR > 0) w.y = -w.y;
+                        v[q * R + r] = cmul(v[q * R + r], w);
+                    }
+                }
+                Radix<R>::template dft<DIR>(v + q * R);
+            }
+        }
+        if constexpr (I + 1 < NP) {
+#pragma unroll
+            for (int q = 0; q < B; q++) {
+                const int jj = t + q * TPP;
+                if (NB % TPP == 0 || jj < NB) {
+                    const int base = (jj / Ns) * Ns * R + (jj % Ns);
+#pragma unroll
+                    for (int r = 0; r < R; r++) lds[Addr::at(p, base + r * Ns)] = v[q * R + r];
+                }
+            }
+            __syncthreads();
+            constexpr int R2 = Info::rad[I + 1];
+            constexpr int NB2 = N / R2;
+            constexpr int B2 = (NB2 + TPP - 1) / TPP;
+#pragma unroll
+            for (int q = 0; q < B2; q++) {
+                const int jj = t + q * TPP;
+                if (NB2 % TPP == 0 || jj < NB2) {
+#pragma unroll
+                    for (int r = 0; r < R2; r++) v[q * R2 + r] = lds[Addr::at(p, jj + r * NB2)];
+                }
+            }
+            __syncthreads();
+            pass<I + 1>(v, lds, tw, t, p);
+        }
+    }
+
+    // Full transform: v holds the first-pass input pattern on entry and the
+    // last-pass output pattern on exit.  All threads of the block must call.
+    __device__ __forceinline__ static void run(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t, int p) {
+        pass<0>(v, lds, tw, t, p);
+    }
+};
+
+}  // namespace ns3d

@@ -1,0 +1,496 @@
+// k_fused.hpp -- the fused pseudo-spectral stage pipeline.
+//
+// One evaluation of the 2/3-masked nonlinear term (reference
+// calculate_convolution_2p3, Source/advection_2_3.cu:480-497) plus the
+// projection/diffusion/update epilogue (Source/diffusion.cu:96-365) runs as
+// five passes over HBM:
+//   P1 x-inverse : w^ (3, FP64 state) -> mask, i kx  -> A (6 fields)
+//   P2 y-inverse : A (6)              -> i ky       -> B (9 fields)
+//   P3 z         : B (9) -> C2R (+ i kz), (w.grad)w, R2C -> C (3 fields)
+//   P4 y-forward : C (3) -> D (3)
+//   P5 x-forward : D (3) -> IPQ -> epilogue (update / RHS / Q out)
+// Real-space fields never touch HBM.  NF=2 evaluates B(u,v) of the J.v path
+// (calculate_convolution_2p3_UV, :500-525) with the same passes.
+#pragma once
+#include "fft_core.hpp"
+
+namespace ns3d {
+
+template <typename T, int N> struct PtrArr { T* p[N]; };
+
+// ---------------------------------------------------------------- P1
+// x-inverse of M w^_b and of M (i kx w^_b).  Block: one y index k and P
+// consecutive z indices l; lanes p-fastest so each row load is contiguous.
+template <typename TB, int NX, int P, int NF>
+__global__ void __launch_bounds__(P* Plan<NX>::TPP)
+k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int nlt,
+             SpecTables tab, const cplx<TB>* __restrict__ tw) {
+    constexpr int TPP = Plan<NX>::TPP;
+    using F = PencilFFT<TB, NX, TPP, +1, typename Plan<NX>::R, IndexMajor<P>>;
+    __shared__ cplx<TB> lds[NX * P];
+    const int tid = threadIdx.x, p = tid % P, t = tid / P;
+    const int k = blockIdx.x / nlt, lt = blockIdx.x % nlt;
+    const int l = lt * P + p;
+    const bool lv = l < Mz;
+    const long S = (long)Ny * Mz;
+    const long base = (long)k * Mz + l;
+#pragma unroll 1
+    for (int f = 0; f < 3 * NF; f++) {
+        const cplx<double>* __restrict__ src = w.p[f];
+        cplx<TB> v[F::E], d[F::E];
+#pragma unroll
+        for (int q = 0; q < F::B0; q++) {
+#pragma unroll
+            for (int r = 0; r < F::R0; r++) {
+                const int j = F::in_index(t, q, r);
+                cplx<double> s = mk<double>(0.0, 0.0);
+                double kx = 0.0;
+                if (F::in_active(t, q)) {
+                    kx = tab.kx[j];
+                    if (lv && mask_keep(tab, j, k, l)) s = src[base + j * S];
+                }
+                v[q * F::R0 + r] = cvt<TB>(s);
+                d[q * F::R0 + r] = mk<TB>((TB)(-s.y * kx), (TB)(s.x * kx));
+            }
+        }
+        const int s_ = f / 3, b = f % 3;
+        cplx<TB>* __restrict__ ow = A + (long)(s_ * 6 + b) * nc;
+        cplx<TB>* __restrict__ od = A + (long)(s_ * 6 + 3 + b) * nc;
+        F::run(v, lds, tw, t, p);
+        F::run(d, lds, tw, t, p);
+        if (lv) {
+#pragma unroll
+            for (int q = 0; q < F::BL; q++) {
+                if (!F::out_active(t, q)) continue;
+#pragma unroll
+                for (int r = 0; r < F::RL; r++) {
+                    const long g = base + F::out_index(t, q, r) * S;
+                    ow[g] = v[q * F::RL + r];
+                    od[g] = d[q * F::RL + r];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- P2
+// y-inverse: B(w_b) , B(dx w_b) , B(i ky w_b).  Block: one x index j and P z indices.
+template <typename TB, int NY, int P, int NF>
+__global__ void __launch_bounds__(P* Plan<NY>::TPP)
+k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, int Ny, int Mz, int nlt,
+             SpecTables tab, const cplx<TB>* __restrict__ tw) {
+    constexpr int TPP = Plan<NY>::TPP;
+    using F = PencilFFT<TB, NY, TPP, +1, typename Plan<NY>::R, IndexMajor<P>>;
+    __shared__ cplx<TB> lds[NY * P];
+    const int tid = threadIdx.x, p = tid % P, t = tid / P;
+    const int j = blockIdx.x / nlt, lt = blockIdx.x % nlt;
+    const int l = lt * P + p;
+    const bool lv = l < Mz;
+    const long S = Mz;
+    const long base = (long)j * Ny * Mz + l;
+#pragma unroll 1
+    for (int f = 0; f < 3 * NF; f++) {
+        const int s_ = f / 3, b = f % 3;
+        // velocity component: produce w_b and dy w_b
+        {
+            const cplx<TB>* __restrict__ src = A + (long)(s_ * 6 + b) * nc;
+            cplx<TB> v[F::E], d[F::E];
+#pragma unroll
+            for (int q = 0; q < F::B0; q++) {
+#pragma unroll
+                for (int r = 0; r < F::R0; r++) {
+                    const int kk = F::in_index(t, q, r);
+                    cplx<TB> s = mk<TB>(0, 0);
+                    TB ky = 0;
+                    if (F::in_active(t, q)) {
+                        ky = (TB)tab.ky[kk];
+                        if (lv) s = src[base + kk * S];
+                    }
+                    v[q * F::R0 + r] = s;
+                    d[q * F::R0 + r] = mk<TB>(-s.y * ky, s.x * ky);
+                }
+            }
+            F::run(v, lds, tw, t, p);
+            F::run(d, lds, tw, t, p);
+            cplx<TB>* __restrict__ ow = B + (long)(s_ * 9 + b) * nc;
+            cplx<TB>* __restrict__ od = B + (long)(s_ * 9 + 6 + b) * nc;
+            if (lv) {
+#pragma unroll
+                for (int q = 0; q < F::BL; q++) {
+                    if (!F::out_active(t, q)) continue;
+#pragma unroll
+                    for (int r = 0; r < F::RL; r++) {
+                        const long g = base + F::out_index(t, q, r) * S;
+                        ow[g] = v[q * F::RL + r];
+                        od[g] = d[q * F::RL + r];
+                    }
+                }
+            }
+        }
+        // x-derivative component: plain y-inverse
+        {
+            const cplx<TB>* __restrict__ src = A + (long)(s_ * 6 + 3 + b) * nc;
+            cplx<TB> v[F::E];
+#pragma unroll
+            for (int q = 0; q < F::B0; q++) {
+#pragma unroll
+                for (int r = 0; r < F::R0; r++) {
+                    const int kk = F::in_index(t, q, r);
+                    cplx<TB> s = mk<TB>(0, 0);
+                    if (F::in_active(t, q) && lv) s = src[base + kk * S];
+                    v[q * F::R0 + r] = s;
+                }
+            }
+            F::run(v, lds, tw, t, p);
+            cplx<TB>* __restrict__ ow = B + (long)(s_ * 9 + 3 + b) * nc;
+            if (lv) {
+#pragma unroll
+                for (int q = 0; q < F::BL; q++) {
+                    if (!F::out_active(t, q)) continue;
+#pragma unroll
+                    for (int r = 0; r < F::RL; r++) ow[base + F::out_index(t, q, r) * S] = v[q * F::RL + r];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- P3
+// z pass on one (x,y) pencil: paired C2R transforms (two real fields per
+// complex FFT), the advective product, and paired R2C transforms.
+// C2R convention: x,y passes already done; only Re of l=0 and l=Nz/2 is used.
+template <typename TB, int NZ, int P>
+struct ZPencil {
+    static constexpr int TPP = Plan<NZ>::TPP;
+    static constexpr int PAD = (sizeof(TB) == 8) ? 1 : 2;
+    using LA = PencilMajor<NZ, PAD>;
+    using FI = PencilFFT<TB, NZ, TPP, +1, typename Plan<NZ>::R, LA>;
+    using FF = PencilFFT<TB, NZ, TPP, -1, typename SeqRev<typename Plan<NZ>::R>::type, LA>;
+    static_assert(FI::RL == FF::R0 && FI::BL == FF::B0, "mirror plans must share the product layout");
+    static constexpr int E = FI::E > FF::E ? FI::E : FF::E;
+    static constexpr int NPT = FI::BL * FI::RL;  // physical points per thread
+
+    // Load the Hermitian extension of (X1 + i X2); dk selects i*kz multiplication.
+    __device__ __forceinline__ static void load_pair(cplx<TB>* v, const cplx<TB>* __restrict__ s1, bool d1,
+                                                     const cplx<TB>* __restrict__ s2, bool d2, bool valid,
+                                                     const SpecTables& tab, int t) {
+#pragma unroll
+        for (int q = 0; q < FI::B0; q++) {
+#pragma unroll
+            for (int r = 0; r < FI::R0; r++) {
+                const int n = FI::in_index(t, q, r);
+                const int sl = n <= NZ / 2 ? n : NZ - n;
+                cplx<TB> a = mk<TB>(0, 0), b = mk<TB>(0, 0);
+                TB kz = 0;
+                if (FI::in_active(t, q)) {
+                    kz = (TB)tab.kz[sl];
+                    if (valid) {
+                        a = s1[sl];
+                        b = s2[sl];
+                    }
+                }
+                if (d1) a = mk<TB>(-a.y * kz, a.x * kz);
+                if (d2) b = mk<TB>(-b.y * kz, b.x * kz);
+                if (n == 0 || n == NZ / 2) {
+                    a.y = 0;
+                    b.y = 0;
+                } else if (n > NZ / 2) {
+                    a.y = -a.y;
+                    b.y = -b.y;
+                }
+                v[q * FI::R0 + r] = mk<TB>(a.x - b.y, a.y + b.x);
+            }
+        }
+    }
+
+    // Inverse-transform a pair; x1/x2 receive the scaled real fields.
+    __device__ __forceinline__ static void c2r_pair(TB* x1, TB* x2, const cplx<TB>* s1, bool d1, const cplx<TB>* s2, bool d2,
+                                                    bool valid, const SpecTables& tab, cplx<TB>* lds,
+                                                    const cplx<TB>* tw, int t, int p, TB scale) {
+        cplx<TB> v[E];
+        load_pair(v, s1, d1, s2, d2, valid, tab, t);
+        FI::run(v, lds, tw, t, p);
+#pragma unroll
+        for (int e = 0; e < NPT; e++) {
+            x1[e] = v[e].x * scale;
+            x2[e] = v[e].y * scale;
+        }
+    }
+
+    // Forward-transform (q1 + i q2) and store the two half spectra (q2 may be null).
+    __device__ __forceinline__ static void r2c_pair(const TB* q1, const TB* q2, cplx<TB>* __restrict__ o1,
+                                                    cplx<TB>* __restrict__ o2, bool valid, cplx<TB>* lds,
+                                                    const cplx<TB>* tw, int t, int p) {
+        cplx<TB> v[E];
+#pragma unroll
+        for (int e = 0; e < NPT; e++) v[e] = mk<TB>(q1[e], q2 ? q2[e] : (TB)0);
+        FF::run(v, lds, tw, t, p);
+#pragma unroll
+        for (int q = 0; q < FF::BL; q++) {
+            if (!FF::out_active(t, q)) continue;
+#pragma unroll
+            for (int r = 0; r < FF::RL; r++) lds[LA::at(p, FF::out_index(t, q, r))] = v[q * FF::RL + r];
+        }
+        __syncthreads();
+        if (valid) {
+            for (int n = t; n <= NZ / 2; n += TPP) {
+                const cplx<TB> zn = lds[LA::at(p, n)];
+                if (q2) {
+                    const cplx<TB> zm = lds[LA::at(p, (NZ - n) % NZ)];
+                    o1[n] = mk<TB>((TB)0.5 * (zn.x + zm.x), (TB)0.5 * (zn.y - zm.y));
+                    o2[n] = mk<TB>((TB)0.5 * (zn.y + zm.y), (TB)0.5 * (zm.x - zn.x));
+                } else {
+                    o1[n] = zn;
+                }
+            }
+        }
+        __syncthreads();
+    }
+};
+
+template <typename TB, int NZ, int P, int NF>
+__global__ void __launch_bounds__(P* Plan<NZ>::TPP)
+k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, long npen, int Mz, SpecTables tab,
+            const cplx<TB>* __restrict__ tw, TB scale) {
+    using Z = ZPencil<TB, NZ, P>;
+    constexpr int TPP = Z::TPP, NPT = Z::NPT;
+    __shared__ cplx<TB> lds[P * (NZ + Z::PAD)];
+    const int tid = threadIdx.x, t = tid % TPP, p = tid / TPP;
+    const long pen = (long)blockIdx.x * P + p;
+    const bool valid = pen < npen;
+    const long off = valid ? pen * Mz : 0;
+    auto fld = [&](int f) { return Bf + (long)f * nc + off; };
+    // field ids: s*9 + {0..2: w_b, 3..5: dx w_b, 6..8: dy w_b}
+    TB Q[3][NPT];
+    if constexpr (NF == 1) {
+        TB w[3][NPT], a[NPT], b[NPT], keep[NPT];
+        Z::c2r_pair(w[0], w[1], fld(0), false, fld(1), false, valid, tab, lds, tw, t, p, scale);
+        Z::c2r_pair(w[2], keep, fld(2), false, fld(3), false, valid, tab, lds, tw, t, p, scale);  // w_z, dx w_x
+        Z::c2r_pair(a, b, fld(6), false, fld(0), true, valid, tab, lds, tw, t, p, scale);         // dy w_x, dz w_x
+#pragma unroll
+        for (int e = 0; e < NPT; e++) Q[0][e] = w[0][e] * keep[e] + w[1][e] * a[e] + w[2][e] * b[e];
+        Z::c2r_pair(keep, a, fld(4), false, fld(7), false, valid, tab, lds, tw, t, p, scale);     // dx w_y, dy w_y
+        Z::c2r_pair(b, Q[2], fld(1), true, fld(5), false, valid, tab, lds, tw, t, p, scale);      // dz w_y, dx w_z
+#pragma unroll
+        for (int e = 0; e < NPT; e++) Q[1][e] = w[0][e] * keep[e] + w[1][e] * a[e] + w[2][e] * b[e];
+        Z::c2r_pair(a, b, fld(8), false, fld(2), true, valid, tab, lds, tw, t, p, scale);         // dy w_z, dz w_z
+#pragma unroll
+        for (int e = 0; e < NPT; e++) Q[2][e] = w[0][e] * Q[2][e] + w[1][e] * a[e] + w[2][e] * b[e];
+    } else {
+        // Q_b = (v.grad) u_b + (u.grad) v_b  (convolute_2p3_UV, Source/advection_2_3.cu:401-475)
+        TB u[3][NPT], vv[3][NPT], a[NPT], b[NPT];
+        Z::c2r_pair(u[0], u[1], fld(0), false, fld(1), false, valid, tab, lds, tw, t, p, scale);
+        Z::c2r_pair(u[2], vv[0], fld(2), false, fld(9), false, valid, tab, lds, tw, t, p, scale);
+        Z::c2r_pair(vv[1], vv[2], fld(10), false, fld(11), false, valid, tab, lds, tw, t, p, scale);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            // (v.grad) u_c
+            Z::c2r_pair(a, b, fld(3 + c), false, fld(6 + c), false, valid, tab, lds, tw, t, p, scale);
+#pragma unroll
+            for (int e = 0; e < NPT; e++) Q[c][e] = vv[0][e] * a[e] + vv[1][e] * b[e];
+            Z::c2r_pair(a, b, fld(c), true, fld(9 + 3 + c), false, valid, tab, lds, tw, t, p, scale);  // dz u_c, dx v_c
+#pragma unroll
+            for (int e = 0; e < NPT; e++) Q[c][e] = Q[c][e] + vv[2][e] * a[e];
+            TB cdx[NPT];
+#pragma unroll
+            for (int e = 0; e < NPT; e++) cdx[e] = b[e];
+            // (u.grad) v_c
+            Z::c2r_pair(a, b, fld(9 + 6 + c), false, fld(9 + c), true, valid, tab, lds, tw, t, p, scale);  // dy v_c, dz v_c
+#pragma unroll
+            for (int e = 0; e < NPT; e++) Q[c][e] = Q[c][e] + (u[0][e] * cdx[e] + u[1][e] * a[e] + u[2][e] * b[e]);
+        }
+    }
+    cplx<TB>* o0 = C + off;
+    cplx<TB>* o1 = C + nc + off;
+    cplx<TB>* o2 = C + 2 * nc + off;
+    Z::r2c_pair(Q[0], Q[1], o0, o1, valid, lds, tw, t, p);
+    Z::r2c_pair(Q[2], nullptr, o2, nullptr, valid, lds, tw, t, p);
+}
+
+// ---------------------------------------------------------------- P4 / generic
+// C2C transform along x (AXIS 0) or y (AXIS 1) of nf spectral fields, out of
+// place or in place (each block reads its pencils completely before writing).
+template <typename TB, int NA, int P, int DIR, int AXIS>
+__global__ void __launch_bounds__(P* Plan<NA>::TPP)
+k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, int nlt,
+           const cplx<TB>* __restrict__ tw) {
+    constexpr int TPP = Plan<NA>::TPP;
+    using F = PencilFFT<TB, NA, TPP, DIR, typename Plan<NA>::R, IndexMajor<P>>;
+    __shared__ cplx<TB> lds[NA * P];
+    const int tid = threadIdx.x, p = tid % P, t = tid / P;
+    const int o = blockIdx.x / nlt, lt = blockIdx.x % nlt;
+    const int l = lt * P + p;
+    const bool lv = l < Mz;
+    const long S = AXIS == 0 ? (long)Ny * Mz : (long)Mz;
+    const long base = AXIS == 0 ? (long)o * Mz + l : (long)o * Ny * Mz + l;
+#pragma unroll 1
+    for (int f = 0; f < nf; f++) {
+        const cplx<TB>* src = in + (long)f * nc;
+        cplx<TB>* dst = out + (long)f * nc;
+        cplx<TB> v[F::E];
+#pragma unroll
+        for (int q = 0; q < F::B0; q++) {
+#pragma unroll
+            for (int r = 0; r < F::R0; r++) {
+                cplx<TB> s = mk<TB>(0, 0);
+                if (F::in_active(t, q) && lv) s = src[base + F::in_index(t, q, r) * S];
+                v[q * F::R0 + r] = s;
+            }
+        }
+        __syncthreads();  // in-place use: every pencil element is read before any is written
+        F::run(v, lds, tw, t, p);
+        if (lv) {
+#pragma unroll
+            for (int q = 0; q < F::BL; q++) {
+                if (!F::out_active(t, q)) continue;
+#pragma unroll
+                for (int r = 0; r < F::RL; r++) dst[base + F::out_index(t, q, r) * S] = v[q * F::RL + r];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- P5 epilogues
+enum EpiMode { EPI_Q = 0, EPI_STAGE = 1, EPI_RHS = 2, EPI_UV = 3, EPI_UV_RHS = 4, EPI_IUV = 5 };
+
+struct EpiArgs {
+    cplx<double>* w[3];          // state (read; written in place for UV/UV_RHS/IUV)
+    const cplx<double>* u0[3];   // SSP combination base (EPI_STAGE, may be null)
+    const cplx<double>* f[3];    // forcing (EPI_STAGE / EPI_RHS, may be null)
+    cplx<double>* out[3];        // output (EPI_Q, EPI_STAGE, EPI_RHS)
+    double dt, Re, a0, a1;       // EPI_STAGE: out = a0*u0 + a1*E(w)
+    int ipq;                     // reference IPQ rule (Q1)
+};
+
+// Projection/diffusion/update at one mode (Source/diffusion.cu:96-365).
+template <int MODE>
+__device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& tab, int j, int k, int l, long g,
+                                         cplx<double> Qx, cplx<double> Qy, cplx<double> Qz) {
+    if (MODE == EPI_Q) {
+        ea.out[0][g] = Qx;
+        ea.out[1][g] = Qy;
+        ea.out[2][g] = Qz;
+        return;
+    }
+    const ProjCoef c = proj_coef(tab, j, k, l);
+    const double AQr[3] = {c.a11 * Qx.x + c.a12 * Qy.x + c.a13 * Qz.x, c.a12 * Qx.x + c.a22 * Qy.x + c.a23 * Qz.x,
+                           c.a13 * Qx.x + c.a23 * Qy.x + c.a33 * Qz.x};
+    const double AQi[3] = {c.a11 * Qx.y + c.a12 * Qy.y + c.a13 * Qz.y, c.a12 * Qx.y + c.a22 * Qy.y + c.a23 * Qz.y,
+                           c.a13 * Qx.y + c.a23 * Qy.y + c.a33 * Qz.y};
+    const cplx<double> Q[3] = {Qx, Qy, Qz};
+    const double dt = ea.dt, Re = ea.Re;
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        const cplx<double> w = ea.w[b][g];
+        cplx<double> r;
+        if (MODE == EPI_STAGE) {
+            const double G = (1.0 - c.din * dt / Re);
+            cplx<double> f = mk<double>(0.0, 0.0);
+            if (ea.f[b]) f = ea.f[b][g];
+            r.x = dt * AQr[b] + G * w.x + dt * f.x - dt * Q[b].x;
+            r.y = dt * AQi[b] + G * w.y + dt * f.y - dt * Q[b].y;
+            if (ea.u0[b]) {
+                const cplx<double> u0 = ea.u0[b][g];
+                r.x = ea.a0 * u0.x + ea.a1 * r.x;
+                r.y = ea.a0 * u0.y + ea.a1 * r.y;
+            }
+            ea.out[b][g] = r;
+        } else if (MODE == EPI_RHS) {
+            const double G = (-c.din / Re);
+            cplx<double> f = mk<double>(0.0, 0.0);
+            if (ea.f[b]) f = ea.f[b][g];
+            r.x = AQr[b] + G * w.x + f.x - Q[b].x;
+            r.y = AQi[b] + G * w.y + f.y - Q[b].y;
+            ea.out[b][g] = r;
+        } else if (MODE == EPI_UV) {
+            const double G = (1.0 - c.din * dt / Re);
+            r.x = dt * AQr[b] + G * w.x - dt * Q[b].x;
+            r.y = dt * AQi[b] + G * w.y - dt * Q[b].y;
+            ea.w[b][g] = r;
+        } else if (MODE == EPI_UV_RHS) {
+            const double G = (-c.din / Re);
+            r.x = AQr[b] + G * w.x - Q[b].x;
+            r.y = AQi[b] + G * w.y - Q[b].y;
+            ea.w[b][g] = r;
+        } else {  // EPI_IUV
+            const double G = (1.0 + c.din * dt / Re);
+            r.x = (dt * AQr[b] + w.x - dt * Q[b].x) / G;
+            r.y = (dt * AQi[b] + w.y - dt * Q[b].y) / G;
+            ea.w[b][g] = r;
+        }
+    }
+}
+
+// set_to_zero_real (Source/advection_2_3.cu:35-70)
+__device__ __forceinline__ void ipq_rule(cplx<double>& Qx, cplx<double>& Qy, cplx<double>& Qz, int ipq, bool origin) {
+    if (ipq) {
+        Qx.x = 0.0;
+        Qy.x = 0.0;
+        Qz.x = 0.0;
+        if (origin) {
+            Qx.y = 0.0;
+            Qy.y = 0.0;
+            Qz.y = 0.0;
+        }
+    }
+}
+
+template <typename TB, int NX, int P, int MODE>
+__global__ void __launch_bounds__(P* Plan<NX>::TPP)
+k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int nlt, SpecTables tab,
+            const cplx<TB>* __restrict__ tw, EpiArgs ea) {
+    constexpr int TPP = Plan<NX>::TPP;
+    using F = PencilFFT<TB, NX, TPP, -1, typename Plan<NX>::R, IndexMajor<P>>;
+    __shared__ cplx<TB> lds[NX * P];
+    const int tid = threadIdx.x, p = tid % P, t = tid / P;
+    const int k = blockIdx.x / nlt, lt = blockIdx.x % nlt;
+    const int l = lt * P + p;
+    const bool lv = l < Mz;
+    const long S = (long)Ny * Mz;
+    const long base = (long)k * Mz + l;
+    cplx<TB> qv[3][F::E];
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        const cplx<TB>* src = D + (long)b * nc;
+#pragma unroll
+        for (int q = 0; q < F::B0; q++) {
+#pragma unroll
+            for (int r = 0; r < F::R0; r++) {
+                cplx<TB> s = mk<TB>(0, 0);
+                if (F::in_active(t, q) && lv) s = src[base + F::in_index(t, q, r) * S];
+                qv[b][q * F::R0 + r] = s;
+            }
+        }
+        F::run(qv[b], lds, tw, t, p);
+    }
+    if (!lv) return;
+#pragma unroll
+    for (int q = 0; q < F::BL; q++) {
+        if (!F::out_active(t, q)) continue;
+#pragma unroll
+        for (int r = 0; r < F::RL; r++) {
+            const int j = F::out_index(t, q, r);
+            const long g = base + (long)j * S;
+            cplx<double> Qx = cvt<double>(qv[0][q * F::RL + r]);
+            cplx<double> Qy = cvt<double>(qv[1][q * F::RL + r]);
+            cplx<double> Qz = cvt<double>(qv[2][q * F::RL + r]);
+            ipq_rule(Qx, Qy, Qz, ea.ipq, (j | k | l) == 0);
+            epilogue<MODE>(ea, tab, j, k, l, g, Qx, Qy, Qz);
+        }
+    }
+}
+
+// Pointwise epilogue from stored Q^ (the standalone solve_advection_diffusion_projection*).
+template <int MODE>
+__global__ void __launch_bounds__(256)
+k_update(const cplx<double>* __restrict__ Qx, const cplx<double>* __restrict__ Qy, const cplx<double>* __restrict__ Qz,
+         long nc, int Ny, int Mz, SpecTables tab, EpiArgs ea) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nc) return;
+    const int l = (int)(g % Mz);
+    const long t1 = g / Mz;
+    const int k = (int)(t1 % Ny), j = (int)(t1 / Ny);
+    epilogue<MODE>(ea, tab, j, k, l, g, Qx[g], Qy[g], Qz[g]);
+}
+
+}  // namespace ns3d

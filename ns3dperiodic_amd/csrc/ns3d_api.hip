@@ -1,0 +1,1014 @@
+// ns3d_api.hip -- context, launch plumbing and the extern "C" entry points of
+// include/ns3d.h.  One translation unit; compiled for gfx950 only.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/ns3d.h"
+#include "k_misc.hpp"
+
+using namespace ns3d;
+
+namespace {
+
+typedef cplx<double> zd;
+
+template <class F>
+int with_size(int n, F&& f) {
+    switch (n) {
+        case 8: return f(std::integral_constant<int, 8>{});
+        case 16: return f(std::integral_constant<int, 16>{});
+        case 32: return f(std::integral_constant<int, 32>{});
+        case 48: return f(std::integral_constant<int, 48>{});
+        case 64: return f(std::integral_constant<int, 64>{});
+        case 96: return f(std::integral_constant<int, 96>{});
+        case 128: return f(std::integral_constant<int, 128>{});
+        case 192: return f(std::integral_constant<int, 192>{});
+        case 256: return f(std::integral_constant<int, 256>{});
+        case 512: return f(std::integral_constant<int, 512>{});
+        default: return NS3D_ERR_SIZE;
+    }
+}
+
+bool size_ok(int n) {
+    return n == 8 || n == 16 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 192 || n == 256 ||
+           n == 512;
+}
+
+// pencils per block in the x/y passes: rows of 128 B
+template <typename TB> constexpr int PXY = sizeof(TB) == 8 ? 8 : 16;
+// pencils per block in the z pass: ~128 threads
+template <int N> constexpr int PZ = (128 / Plan<N>::TPP) > 0 ? (128 / Plan<N>::TPP) : 1;
+
+#define HIPCHK(x)                                         \
+    do {                                                  \
+        hipError_t e_ = (x);                              \
+        if (e_ != hipSuccess) return NS3D_ERR_HIP;        \
+    } while (0)
+
+}  // namespace
+
+struct ns3d_ctx {
+    int Nx, Ny, Nz, Mz;
+    long nc, nr, npen;
+    double Lx, Ly, Lz, Re, dt;
+    int precision;
+    unsigned flags;
+    hipStream_t stream;
+    int device;
+    int last_error;
+    // device tables
+    double* dtab = nullptr;
+    SpecTables tab;
+    void* tw64[3] = {nullptr, nullptr, nullptr};  // x, y, z twiddles (double)
+    void* tw32[3] = {nullptr, nullptr, nullptr};  // (float)
+    // bulk scratch (A|B regions) in bulk precision; capacity in fields
+    void* scratch = nullptr;
+    int scratch_fields = 0;
+    // FP64 stage buffers and a Q^ buffer
+    zd* stage = nullptr;  // 9 fields: u1[3], u2[3], q[3]
+    zd* extra = nullptr;  // 6 fields: tangent map v1[3] + spare
+    double* red = nullptr;  // reduction partials (2*RED_BLOCKS + 2)
+    double* red_host = nullptr;  // pinned
+    void* real_scratch = nullptr;  // physical-space scratch (FP64 or FP32), 4 fields
+    // registered base flow for the Krylov callbacks
+    zd* base[3] = {nullptr, nullptr, nullptr};
+    const zd* bforce[3] = {nullptr, nullptr, nullptr};
+    int timesteps_period = 1;
+    double tau = 1.0;
+    zd* kry = nullptr;  // 3 fields for J.v unpacking
+};
+
+namespace {
+
+int set_err(ns3d_ctx* c, int s) {
+    if (c) c->last_error = s;
+    return s;
+}
+
+// ---------------------------------------------------------------- host tables
+int build_tables(ns3d_ctx* c) {
+    const int Nx = c->Nx, Ny = c->Ny, Mz = c->Mz;
+    const double Lx = c->Lx, Ly = c->Ly, Lz = c->Lz;
+    const double PI = 3.141592653589793238462643383279502884;  // reference PI literal (Source/Macros.h:75)
+    const int tot = 5 * (Nx + Ny + Mz);
+    std::vector<double> h(tot);
+    double* kx = h.data();
+    double* ky = kx + Nx;
+    double* kz = ky + Ny;
+    double* sx = kz + Mz;
+    double* sy = sx + Nx;
+    double* sz = sy + Ny;
+    double* px = sz + Mz;
+    double* py = px + Nx;
+    double* pz = py + Ny;
+    double* fm = pz + Mz;
+    double* fn = fm + Nx;
+    double* fq = fn + Ny;
+    double* mx = fq + Mz;
+    double* my = mx + Nx;
+    double* mz = my + Ny;
+    (void)mz;
+    std::vector<double> mxv(Nx), myv(Ny), mzv(Mz);
+    const double k1 = 2.0 * M_PI / Lx, k2 = 2.0 * M_PI / Ly, k3 = 2.0 * M_PI / Lz;
+    const double kxMax = 2 * Nx * PI / Lx, kyMax = 2 * Ny * PI / Ly, kzMax = 2 * Mz * PI / Lz;
+    for (int j = 0; j < Nx; j++) {
+        const int m = j >= Nx / 2 ? j - Nx : j;
+        kx[j] = 2.0 * M_PI / Lx * m;            // build_Nabla_Wavenumbers (math_support.cu:393-414)
+        sx[j] = kx[j] * kx[j];                  // build_Laplace_and_Diffusion (:374-390)
+        px[j] = k1 * k1 * m * m;                // build_projection_matrix_elements (:418-466)
+        fm[j] = (double)m;
+        const double km = m * 2 * PI / Lx;      // build_mask_matrix (:471-519)
+        mxv[j] = km * km / (kxMax * kxMax);
+    }
+    for (int k = 0; k < Ny; k++) {
+        const int n = k >= Ny / 2 ? k - Ny : k;
+        ky[k] = 2.0 * M_PI / Ly * n;
+        sy[k] = ky[k] * ky[k];
+        py[k] = k2 * k2 * n * n;
+        fn[k] = (double)n;
+        const double kn = n * 2 * PI / Ly;
+        myv[k] = kn * kn / (kyMax * kyMax);
+    }
+    for (int l = 0; l < Mz; l++) {
+        const int q = l;
+        kz[l] = 2.0 * M_PI / Lz * q;
+        sz[l] = kz[l] * kz[l];
+        pz[l] = k3 * k3 * q * q;
+        fq[l] = (double)q;
+        const double kq = q * 2 * PI / Lz;
+        mzv[l] = kq * kq / (kzMax * kzMax);
+    }
+    std::vector<double> all(h.begin(), h.begin() + 4 * (Nx + Ny + Mz));
+    all.insert(all.end(), mxv.begin(), mxv.end());
+    all.insert(all.end(), myv.begin(), myv.end());
+    all.insert(all.end(), mzv.begin(), mzv.end());
+    HIPCHK(hipMalloc(&c->dtab, sizeof(double) * all.size()));
+    HIPCHK(hipMemcpy(c->dtab, all.data(), sizeof(double) * all.size(), hipMemcpyHostToDevice));
+    const double* d = c->dtab;
+    SpecTables& t = c->tab;
+    t.kx = d; d += Nx; t.ky = d; d += Ny; t.kz = d; d += Mz;
+    t.sx = d; d += Nx; t.sy = d; d += Ny; t.sz = d; d += Mz;
+    t.px = d; d += Nx; t.py = d; d += Ny; t.pz = d; d += Mz;
+    t.fm = d; d += Nx; t.fn = d; d += Ny; t.fq = d; d += Mz;
+    t.mx = d; d += Nx; t.my = d; d += Ny; t.mz = d;
+    t.k1k2 = k1 * k2;
+    t.k1k3 = k1 * k3;
+    t.k2k3 = k2 * k3;
+    // largest s with sqrt(s) < 2/3 (the reference compares sqrt(sum) < 2.0/3.0)
+    const double c23 = 2.0 / 3.0;
+    double s = c23 * c23;
+    while (std::sqrt(s) >= c23) s = std::nextafter(s, 0.0);
+    while (std::sqrt(std::nextafter(s, 1.0)) < c23) s = std::nextafter(s, 1.0);
+    t.mask_smax = s;
+    return NS3D_OK;
+}
+
+int build_twiddles(ns3d_ctx* c) {
+    const int ns[3] = {c->Nx, c->Ny, c->Nz};
+    for (int a = 0; a < 3; a++) {
+        const int n = ns[a];
+        std::vector<zd> h64(n);
+        std::vector<cplx<float>> h32(n);
+        for (int m = 0; m < n; m++) {
+            const long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)n;
+            const long double cr = cosl(ang), si = -sinl(ang);
+            h64[m].x = (double)cr;
+            h64[m].y = (double)si;
+            h32[m].x = (float)cr;
+            h32[m].y = (float)si;
+        }
+        HIPCHK(hipMalloc(&c->tw64[a], sizeof(zd) * n));
+        HIPCHK(hipMalloc(&c->tw32[a], sizeof(cplx<float>) * n));
+        HIPCHK(hipMemcpy(c->tw64[a], h64.data(), sizeof(zd) * n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->tw32[a], h32.data(), sizeof(cplx<float>) * n, hipMemcpyHostToDevice));
+    }
+    return NS3D_OK;
+}
+
+template <typename TB> const cplx<TB>* twp(const ns3d_ctx* c, int axis) {
+    return (const cplx<TB>*)(sizeof(TB) == 8 ? c->tw64[axis] : c->tw32[axis]);
+}
+
+size_t bulk_bytes(const ns3d_ctx* c) { return c->precision == NS3D_FP64 ? sizeof(zd) : sizeof(cplx<float>); }
+
+int ensure_scratch(ns3d_ctx* c, int fields) {
+    if (c->scratch_fields >= fields) return NS3D_OK;
+    if (c->scratch) {
+        hipStreamSynchronize(c->stream);
+        hipFree(c->scratch);
+        c->scratch = nullptr;
+        c->scratch_fields = 0;
+    }
+    if (hipMalloc(&c->scratch, bulk_bytes(c) * (size_t)c->nc * fields) != hipSuccess) return NS3D_ERR_ALLOC;
+    c->scratch_fields = fields;
+    return NS3D_OK;
+}
+
+int ensure_real_scratch(ns3d_ctx* c) {
+    if (c->real_scratch) return NS3D_OK;
+    if (hipMalloc(&c->real_scratch, sizeof(double) * (size_t)c->nr * 4) != hipSuccess) return NS3D_ERR_ALLOC;
+    return NS3D_OK;
+}
+
+int ensure_fp64(ns3d_ctx* c, zd** p, int fields) {
+    if (*p) return NS3D_OK;
+    if (hipMalloc((void**)p, sizeof(zd) * (size_t)c->nc * fields) != hipSuccess) return NS3D_ERR_ALLOC;
+    return NS3D_OK;
+}
+
+inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// ---------------------------------------------------------------- fused pipeline
+// Runs P1..P4 for NF vector fields and leaves D (3 fields) at scratch + 3*nc.
+template <typename TB>
+int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
+    int st = ensure_scratch(c, 15 * nf);
+    if (st) return st;
+    cplx<TB>* A = (cplx<TB>*)c->scratch;
+    cplx<TB>* B = A + (long)6 * nf * c->nc;
+    const long nc = c->nc;
+    hipStream_t s = c->stream;
+    // P1 x-inverse
+    st = with_size(c->Nx, [&](auto NXc) -> int {
+        constexpr int NX = decltype(NXc)::value;
+        constexpr int P = PXY<TB>;
+        const int nlt = (c->Mz + P - 1) / P;
+        dim3 grid(c->Ny * nlt), blk(P * Plan<NX>::TPP);
+        if (nf == 1) {
+            PtrArr<const zd, 3> w{{in[0], in[1], in[2]}};
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, c->tab, twp<TB>(c, 0));
+        } else {
+            PtrArr<const zd, 6> w{{in[0], in[1], in[2], in[3], in[4], in[5]}};
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, c->tab, twp<TB>(c, 0));
+        }
+        return NS3D_OK;
+    });
+    if (st) return st;
+    // P2 y-inverse
+    st = with_size(c->Ny, [&](auto NYc) -> int {
+        constexpr int NY = decltype(NYc)::value;
+        constexpr int P = PXY<TB>;
+        const int nlt = (c->Mz + P - 1) / P;
+        dim3 grid(c->Nx * nlt), blk(P * Plan<NY>::TPP);
+        if (nf == 1)
+            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 1>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, c->tab, twp<TB>(c, 1));
+        else
+            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 2>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, c->tab, twp<TB>(c, 1));
+        return NS3D_OK;
+    });
+    if (st) return st;
+    // P3 z: C2R, product, R2C -> C at A
+    st = with_size(c->Nz, [&](auto NZc) -> int {
+        constexpr int NZ = decltype(NZc)::value;
+        constexpr int P = PZ<NZ>;
+        dim3 grid(nblk(c->npen, P)), blk(P * Plan<NZ>::TPP);
+        const TB scale = (TB)(1.0 / (1.0 * c->Nx * c->Ny * c->Nz));
+        if (nf == 1)
+            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 1>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, c->tab, twp<TB>(c, 2), scale);
+        else
+            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 2>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, c->tab, twp<TB>(c, 2), scale);
+        return NS3D_OK;
+    });
+    if (st) return st;
+    // P4 y-forward C -> D
+    st = with_size(c->Ny, [&](auto NYc) -> int {
+        constexpr int NY = decltype(NYc)::value;
+        constexpr int P = PXY<TB>;
+        const int nlt = (c->Mz + P - 1) / P;
+        dim3 grid(c->Nx * nlt), blk(P * Plan<NY>::TPP);
+        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1>), grid, blk, 0, s, A, A + 3 * nc, 3, nc, c->Ny, c->Mz, nlt, twp<TB>(c, 1));
+        return NS3D_OK;
+    });
+    return st;
+}
+
+template <typename TB, int MODE>
+int run_x_forward_epilogue(ns3d_ctx* c, const EpiArgs& ea) {
+    const cplx<TB>* D = (const cplx<TB>*)c->scratch + 3 * c->nc;
+    return with_size(c->Nx, [&](auto NXc) -> int {
+        constexpr int NX = decltype(NXc)::value;
+        constexpr int P = PXY<TB>;
+        const int nlt = (c->Mz + P - 1) / P;
+        dim3 grid(c->Ny * nlt), blk(P * Plan<NX>::TPP);
+        hipLaunchKernelGGL((k_x_fwd_epi<TB, NX, P, MODE>), grid, blk, 0, c->stream, D, c->nc, c->Ny, c->Mz, nlt, c->tab,
+                           twp<TB>(c, 0), ea);
+        return NS3D_OK;
+    });
+}
+
+EpiArgs make_epi(const ns3d_ctx* c) {
+    EpiArgs ea;
+    memset(&ea, 0, sizeof(ea));
+    ea.dt = c->dt;
+    ea.Re = c->Re;
+    ea.a0 = 0.0;
+    ea.a1 = 1.0;
+    ea.ipq = (c->flags & NS3D_FLAG_NO_IPQ) ? 0 : 1;
+    return ea;
+}
+
+// nonlinear term of nf fields followed by the MODE epilogue
+template <int MODE>
+int fused(ns3d_ctx* c, const zd* const* in, int nf, const EpiArgs& ea) {
+    int st;
+    if (c->precision == NS3D_FP64) {
+        st = run_nonlinear_passes<double>(c, in, nf);
+        if (st) return st;
+        st = run_x_forward_epilogue<double, MODE>(c, ea);
+    } else {
+        st = run_nonlinear_passes<float>(c, in, nf);
+        if (st) return st;
+        st = run_x_forward_epilogue<float, MODE>(c, ea);
+    }
+    if (st) return st;
+    return hipGetLastError() == hipSuccess ? NS3D_OK : NS3D_ERR_HIP;
+}
+
+// E(w) stage: out = a0*u0 + a1*(w + dt*L(w))
+int stage_into(ns3d_ctx* c, const zd* const w[3], const zd* const u0[3], double a0, double a1, zd* const out[3],
+               const zd* const f[3]) {
+    EpiArgs ea = make_epi(c);
+    for (int b = 0; b < 3; b++) {
+        ea.w[b] = (zd*)w[b];
+        ea.u0[b] = u0 ? u0[b] : nullptr;
+        ea.f[b] = f ? f[b] : nullptr;
+        ea.out[b] = out[b];
+    }
+    ea.a0 = a0;
+    ea.a1 = a1;
+    return fused<EPI_STAGE>(c, w, 1, ea);
+}
+
+int rk3_one(ns3d_ctx* c, zd* const u[3], const zd* const f[3]) {
+    int st = ensure_fp64(c, &c->stage, 9);
+    if (st) return st;
+    zd* u1[3] = {c->stage, c->stage + c->nc, c->stage + 2 * c->nc};
+    zd* u2[3] = {c->stage + 3 * c->nc, c->stage + 4 * c->nc, c->stage + 5 * c->nc};
+    const zd* uc[3] = {u[0], u[1], u[2]};
+    const zd* u1c[3] = {u1[0], u1[1], u1[2]};
+    const zd* u2c[3] = {u2[0], u2[1], u2[2]};
+    // RK3_SSP (Source/RK_time_step.cu:310-327):
+    //   u1 = E(u); u1 = E(u1); u2 = 3/4 u + 1/4 u1; u2 = E(u2); u = 1/3 u + 2/3 u2
+    if ((st = stage_into(c, uc, nullptr, 0.0, 1.0, u1, f))) return st;
+    if ((st = stage_into(c, u1c, uc, 3.0 / 4.0, 1.0 / 4.0, u2, f))) return st;
+    if ((st = stage_into(c, u2c, uc, 1.0 / 3.0, 2.0 / 3.0, u, f))) return st;
+    return NS3D_OK;
+}
+
+// ---------------------------------------------------------------- standalone 3D transforms
+template <typename TB>
+int c2r_3d(ns3d_ctx* c, const cplx<TB>* in, TB* out, cplx<TB>* tmp, bool scale) {
+    hipStream_t s = c->stream;
+    const long nc = c->nc;
+    int st = with_size(c->Nx, [&](auto N) -> int {
+        constexpr int NX = decltype(N)::value;
+        constexpr int P = PXY<TB>;
+        const int nlt = (c->Mz + P - 1) / P;
+        hipLaunchKernelGGL((k_axis_c2c<TB, NX, P, +1, 0>), dim3(c->Ny * nlt), dim3(P * Plan<NX>::TPP), 0, s, in, tmp, 1, nc,
+                           c->Ny, c->Mz, nlt, twp<TB>(c, 0));
+        return NS3D_OK;
+    });
+    if (st) return st;
+    st = with_size(c->Ny, [&](auto N) -> int {
+        constexpr int NY = decltype(N)::value;
+        constexpr int P = PXY<TB>;
+        const int nlt = (c->Mz + P - 1) / P;
+        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, +1, 1>), dim3(c->Nx * nlt), dim3(P * Plan<NY>::TPP), 0, s, tmp, tmp, 1,
+                           nc, c->Ny, c->Mz, nlt, twp<TB>(c, 1));
+        return NS3D_OK;
+    });
+    if (st) return st;
+    return with_size(c->Nz, [&](auto N) -> int {
+        constexpr int NZ = decltype(N)::value;
+        constexpr int P = PZ<NZ>;
+        const long npairs = c->npen / 2;
+        const TB sc = scale ? (TB)(1.0 / (1.0 * c->Nx * c->Ny * c->Nz)) : (TB)1;
+        hipLaunchKernelGGL((k_z_c2r<TB, NZ, P>), dim3(nblk(npairs, P)), dim3(P * Plan<NZ>::TPP), 0, s, tmp, out, npairs,
+                           c->Mz, c->tab, twp<TB>(c, 2), sc);
+        return NS3D_OK;
+    });
+}
+
+template <typename TB>
+int r2c_3d(ns3d_ctx* c, const TB* in, cplx<TB>* out) {
+    hipStream_t s = c->stream;
+    const long nc = c->nc;
+    int st = with_size(c->Nz, [&](auto N) -> int {
+        constexpr int NZ = decltype(N)::value;
+        constexpr int P = PZ<NZ>;
+        const long npairs = c->npen / 2;
+        hipLaunchKernelGGL((k_z_r2c<TB, NZ, P>), dim3(nblk(npairs, P)), dim3(P * Plan<NZ>::TPP), 0, s, in, out, npairs,
+                           c->Mz, twp<TB>(c, 2));
+        return NS3D_OK;
+    });
+    if (st) return st;
+    st = with_size(c->Ny, [&](auto N) -> int {
+        constexpr int NY = decltype(N)::value;
+        constexpr int P = PXY<TB>;
+        const int nlt = (c->Mz + P - 1) / P;
+        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1>), dim3(c->Nx * nlt), dim3(P * Plan<NY>::TPP), 0, s, out, out, 1,
+                           nc, c->Ny, c->Mz, nlt, twp<TB>(c, 1));
+        return NS3D_OK;
+    });
+    if (st) return st;
+    return with_size(c->Nx, [&](auto N) -> int {
+        constexpr int NX = decltype(N)::value;
+        constexpr int P = PXY<TB>;
+        const int nlt = (c->Mz + P - 1) / P;
+        hipLaunchKernelGGL((k_axis_c2c<TB, NX, P, -1, 0>), dim3(c->Ny * nlt), dim3(P * Plan<NX>::TPP), 0, s, out, out, 1,
+                           nc, c->Ny, c->Mz, nlt, twp<TB>(c, 0));
+        return NS3D_OK;
+    });
+}
+
+// FP64-interface 3D transforms (FP32 bulk: convert, transform, convert back)
+int c2r_any(ns3d_ctx* c, const zd* in, double* out, bool scale) {
+    int st = ensure_scratch(c, 15);
+    if (st) return st;
+    if (c->precision == NS3D_FP64) {
+        st = c2r_3d<double>(c, in, out, (zd*)c->scratch, scale);
+    } else {
+        cplx<float>* a = (cplx<float>*)c->scratch;
+        cplx<float>* b = a + c->nc;
+        float* r = (float*)(b + c->nc);  // nr floats fit in 2 complex float fields (nr <= 2*nc)
+        hipLaunchKernelGGL(k_d2f_c, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, in, a, c->nc);
+        st = c2r_3d<float>(c, a, r, b, scale);
+        if (st) return st;
+        hipLaunchKernelGGL(k_f2d_r, dim3(nblk(c->nr, 256)), dim3(256), 0, c->stream, r, out, c->nr);
+    }
+    if (st) return st;
+    return hipGetLastError() == hipSuccess ? NS3D_OK : NS3D_ERR_HIP;
+}
+
+int r2c_any(ns3d_ctx* c, const double* in, zd* out) {
+    int st;
+    if (c->precision == NS3D_FP64) {
+        st = r2c_3d<double>(c, in, out);
+    } else {
+        st = ensure_scratch(c, 15);
+        if (st) return st;
+        cplx<float>* a = (cplx<float>*)c->scratch;
+        float* r = (float*)(a + c->nc);
+        hipLaunchKernelGGL(k_d2f_r, dim3(nblk(c->nr, 256)), dim3(256), 0, c->stream, in, r, c->nr);
+        st = r2c_3d<float>(c, r, a);
+        if (st) return st;
+        hipLaunchKernelGGL(k_f2d_c, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, a, out, c->nc);
+    }
+    if (st) return st;
+    return hipGetLastError() == hipSuccess ? NS3D_OK : NS3D_ERR_HIP;
+}
+
+// ---------------------------------------------------------------- reductions
+template <int OP, int SRC>
+int reduce(ns3d_ctx* c, const RedArgs& r, long n, double* host_out) {
+    hipStream_t s = c->stream;
+    int nb = (int)std::min<long>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL((k_reduce_partial<OP, SRC>), dim3(nb), dim3(RED_THREADS), 0, s, r, n, c->red);
+    hipLaunchKernelGGL((k_reduce_final<OP>), dim3(1), dim3(RED_THREADS), 0, s, c->red, nb, c->red + 2 * RED_BLOCKS);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->red_host, c->red + 2 * RED_BLOCKS, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    host_out[0] = c->red_host[0];
+    if (OP == 1) host_out[1] = c->red_host[1];
+    return NS3D_OK;
+}
+
+RedArgs red_args(const ns3d_ctx* c, const double* a, const double* b, const double* d) {
+    RedArgs r;
+    r.a = a;
+    r.b = b;
+    r.c = d;
+    r.G = Grid3{c->Nx, c->Ny, c->Nz};
+    r.dx = c->Lx / c->Nx;
+    r.dy = c->Ly / c->Ny;
+    r.dz = c->Lz / c->Nz;
+    r.p0 = 0.0;
+    r.p1 = c->Re;
+    return r;
+}
+
+}  // namespace
+
+// =====================================================================================
+extern "C" {
+
+const char* ns3d_status_string(int s) {
+    switch (s) {
+        case NS3D_OK: return "ok";
+        case NS3D_ERR_ARG: return "invalid argument";
+        case NS3D_ERR_SIZE: return "grid size has no compiled FFT plan";
+        case NS3D_ERR_ALLOC: return "device allocation failed";
+        case NS3D_ERR_HIP: return "HIP runtime error";
+        case NS3D_ERR_NONFINITE: return "non-finite value";
+        case NS3D_ERR_PRECISION: return "operation not offered at this precision";
+        default: return "unknown status";
+    }
+}
+
+int ns3d_supported_size(int n) { return size_ok(n) ? 1 : 0; }
+
+int ns3d_create(const ns3d_config* cfg, ns3d_ctx** out) {
+    if (!cfg || !out) return NS3D_ERR_ARG;
+    *out = nullptr;
+    if (!size_ok(cfg->Nx) || !size_ok(cfg->Ny) || !size_ok(cfg->Nz)) return NS3D_ERR_SIZE;
+    if (!(cfg->Lx > 0) || !(cfg->Ly > 0) || !(cfg->Lz > 0) || !(cfg->Re > 0)) return NS3D_ERR_ARG;
+    if (cfg->precision != NS3D_FP64 && cfg->precision != NS3D_FP32_BULK) return NS3D_ERR_ARG;
+    if (cfg->device >= 0) HIPCHK(hipSetDevice(cfg->device));
+    ns3d_ctx* c = new (std::nothrow) ns3d_ctx();
+    if (!c) return NS3D_ERR_ALLOC;
+    c->Nx = cfg->Nx;
+    c->Ny = cfg->Ny;
+    c->Nz = cfg->Nz;
+    c->Mz = cfg->Nz / 2 + 1;
+    c->nc = (long)c->Nx * c->Ny * c->Mz;
+    c->nr = (long)c->Nx * c->Ny * c->Nz;
+    c->npen = (long)c->Nx * c->Ny;
+    c->Lx = cfg->Lx;
+    c->Ly = cfg->Ly;
+    c->Lz = cfg->Lz;
+    c->Re = cfg->Re;
+    c->dt = cfg->dt;
+    c->precision = cfg->precision;
+    c->flags = cfg->flags;
+    c->stream = (hipStream_t)cfg->stream;
+    hipGetDevice(&c->device);
+    c->last_error = NS3D_OK;
+    int st = build_tables(c);
+    if (!st) st = build_twiddles(c);
+    if (!st && hipMalloc((void**)&c->red, sizeof(double) * (2 * RED_BLOCKS + 2)) != hipSuccess) st = NS3D_ERR_ALLOC;
+    if (!st && hipHostMalloc((void**)&c->red_host, 2 * sizeof(double)) != hipSuccess) st = NS3D_ERR_ALLOC;
+    if (st) {
+        ns3d_destroy(c);
+        return st;
+    }
+    *out = c;
+    return NS3D_OK;
+}
+
+int ns3d_destroy(ns3d_ctx* c) {
+    if (!c) return NS3D_ERR_ARG;
+    hipStreamSynchronize(c->stream);
+    hipFree(c->dtab);
+    for (int a = 0; a < 3; a++) {
+        hipFree(c->tw64[a]);
+        hipFree(c->tw32[a]);
+    }
+    hipFree(c->scratch);
+    hipFree(c->stage);
+    hipFree(c->extra);
+    hipFree(c->red);
+    hipFree(c->real_scratch);
+    hipFree(c->kry);
+    if (c->red_host) hipHostFree(c->red_host);
+    delete c;
+    return NS3D_OK;
+}
+
+int ns3d_set_params(ns3d_ctx* c, double dt, double Re) {
+    if (!c || !(Re > 0)) return NS3D_ERR_ARG;
+    c->dt = dt;
+    c->Re = Re;
+    return NS3D_OK;
+}
+
+int ns3d_set_stream(ns3d_ctx* c, void* stream) {
+    if (!c) return NS3D_ERR_ARG;
+    c->stream = (hipStream_t)stream;
+    return NS3D_OK;
+}
+
+int ns3d_synchronize(ns3d_ctx* c) {
+    if (!c) return NS3D_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS3D_OK;
+}
+
+int ns3d_last_error(const ns3d_ctx* c) { return c ? c->last_error : NS3D_ERR_ARG; }
+
+long long ns3d_workspace_bytes(const ns3d_ctx* c) {
+    if (!c) return -1;
+    long long b = (long long)bulk_bytes(c) * c->nc * c->scratch_fields;
+    if (c->stage) b += (long long)sizeof(zd) * c->nc * 9;
+    if (c->extra) b += (long long)sizeof(zd) * c->nc * 6;
+    if (c->kry) b += (long long)sizeof(zd) * c->nc * 3;
+    if (c->real_scratch) b += (long long)sizeof(double) * c->nr * 4;
+    return b;
+}
+
+// ---- a4
+int ns3d_fft_r2c(ns3d_ctx* c, const double* in, ns3d_cplx* out) {
+    if (!c || !in || !out) return NS3D_ERR_ARG;
+    return r2c_any(c, in, (zd*)out);
+}
+int ns3d_fft_c2r(ns3d_ctx* c, const ns3d_cplx* in, double* out) {
+    if (!c || !in || !out) return NS3D_ERR_ARG;
+    return c2r_any(c, (const zd*)in, out, true);
+}
+int ns3d_fft_c2r_unscaled(ns3d_ctx* c, const ns3d_cplx* in, double* out) {
+    if (!c || !in || !out) return NS3D_ERR_ARG;
+    return c2r_any(c, (const zd*)in, out, false);
+}
+
+// ---- a5 (on-the-fly table recomposition written out, same device helpers as the kernels)
+}  // extern "C"
+
+namespace {
+__global__ void k_tables(SpecTables tab, long nc, int Nx, int Ny, int Mz, double* din_diffusion, double* din_poisson,
+                         double* a11, double* a22, double* a33, double* a12, double* a13, double* a23, double* mask,
+                         double* kx, double* ky, double* kz) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < Nx && kx) kx[g] = tab.kx[g];
+    if (g < Ny && ky) ky[g] = tab.ky[g];
+    if (g < Mz && kz) kz[g] = tab.kz[g];
+    if (g >= nc) return;
+    const int l = (int)(g % Mz);
+    const long t1 = g / Mz;
+    const int k = (int)(t1 % Ny), j = (int)(t1 / Ny);
+    const ProjCoef c = proj_coef(tab, j, k, l);
+    if (din_diffusion) din_diffusion[g] = c.din;
+    if (din_poisson) din_poisson[g] = g == 0 ? 1.0 : -c.din;
+    if (a11) {
+        a11[g] = c.a11; a22[g] = c.a22; a33[g] = c.a33;
+        a12[g] = c.a12; a13[g] = c.a13; a23[g] = c.a23;
+    }
+    if (mask) mask[g] = mask_keep(tab, j, k, l) ? 1.0 : 0.0;
+}
+}  // namespace
+
+extern "C" {
+
+int ns3d_tables(ns3d_ctx* c, double* kx, double* ky, double* kz, double* din_diffusion, double* din_poisson,
+                double* AM11, double* AM22, double* AM33, double* AM12, double* AM13, double* AM23, double* mask) {
+    if (!c) return NS3D_ERR_ARG;
+    if (AM11 && !(AM22 && AM33 && AM12 && AM13 && AM23)) return NS3D_ERR_ARG;
+    hipLaunchKernelGGL(k_tables, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, c->tab, c->nc, c->Nx, c->Ny, c->Mz,
+                       din_diffusion, din_poisson, AM11, AM22, AM33, AM12, AM13, AM23, mask, kx, ky, kz);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+
+// ---- a3
+int ns3d_nonlinear(ns3d_ctx* c, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz, ns3d_cplx* Qx,
+                   ns3d_cplx* Qy, ns3d_cplx* Qz) {
+    if (!c || !ux || !uy || !uz || !Qx || !Qy || !Qz) return NS3D_ERR_ARG;
+    const zd* in[3] = {(const zd*)ux, (const zd*)uy, (const zd*)uz};
+    EpiArgs ea = make_epi(c);
+    ea.out[0] = (zd*)Qx;
+    ea.out[1] = (zd*)Qy;
+    ea.out[2] = (zd*)Qz;
+    return fused<EPI_Q>(c, in, 1, ea);
+}
+
+int ns3d_bilinear(ns3d_ctx* c, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz, const ns3d_cplx* vx,
+                  const ns3d_cplx* vy, const ns3d_cplx* vz, ns3d_cplx* Qx, ns3d_cplx* Qy, ns3d_cplx* Qz) {
+    if (!c || !ux || !uy || !uz || !vx || !vy || !vz || !Qx || !Qy || !Qz) return NS3D_ERR_ARG;
+    const zd* in[6] = {(const zd*)ux, (const zd*)uy, (const zd*)uz, (const zd*)vx, (const zd*)vy, (const zd*)vz};
+    EpiArgs ea = make_epi(c);
+    ea.out[0] = (zd*)Qx;
+    ea.out[1] = (zd*)Qy;
+    ea.out[2] = (zd*)Qz;
+    return fused<EPI_Q>(c, in, 2, ea);
+}
+
+// ---- a6
+int ns3d_project_update(ns3d_ctx* c, int mode, ns3d_cplx* ux, ns3d_cplx* uy, ns3d_cplx* uz, const ns3d_cplx* fx,
+                        const ns3d_cplx* fy, const ns3d_cplx* fz, const ns3d_cplx* Qx, const ns3d_cplx* Qy,
+                        const ns3d_cplx* Qz, ns3d_cplx* Rx, ns3d_cplx* Ry, ns3d_cplx* Rz) {
+    if (!c || !ux || !uy || !uz || !Qx || !Qy || !Qz) return NS3D_ERR_ARG;
+    EpiArgs ea = make_epi(c);
+    zd* w[3] = {(zd*)ux, (zd*)uy, (zd*)uz};
+    const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
+    for (int b = 0; b < 3; b++) {
+        ea.w[b] = w[b];
+        ea.f[b] = f[b];
+        ea.out[b] = w[b];
+    }
+    dim3 g(nblk(c->nc, 256)), t(256);
+    const zd *qx = (const zd*)Qx, *qy = (const zd*)Qy, *qz = (const zd*)Qz;
+    switch (mode) {
+        case NS3D_UPD_STAGE:
+            hipLaunchKernelGGL(k_update<EPI_STAGE>, g, t, 0, c->stream, qx, qy, qz, c->nc, c->Ny, c->Mz, c->tab, ea);
+            break;
+        case NS3D_UPD_UV_RHS:
+            hipLaunchKernelGGL(k_update<EPI_UV_RHS>, g, t, 0, c->stream, qx, qy, qz, c->nc, c->Ny, c->Mz, c->tab, ea);
+            break;
+        case NS3D_UPD_UV:
+            hipLaunchKernelGGL(k_update<EPI_UV>, g, t, 0, c->stream, qx, qy, qz, c->nc, c->Ny, c->Mz, c->tab, ea);
+            break;
+        case NS3D_UPD_IUV:
+            hipLaunchKernelGGL(k_update<EPI_IUV>, g, t, 0, c->stream, qx, qy, qz, c->nc, c->Ny, c->Mz, c->tab, ea);
+            break;
+        case NS3D_UPD_RHS:
+            if (!Rx || !Ry || !Rz) return NS3D_ERR_ARG;
+            ea.out[0] = (zd*)Rx;
+            ea.out[1] = (zd*)Ry;
+            ea.out[2] = (zd*)Rz;
+            hipLaunchKernelGGL(k_update<EPI_RHS>, g, t, 0, c->stream, qx, qy, qz, c->nc, c->Ny, c->Mz, c->tab, ea);
+            break;
+        default: return NS3D_ERR_ARG;
+    }
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+
+// ---- a7
+int ns3d_copy3(ns3d_ctx* c, const ns3d_cplx* sx, const ns3d_cplx* sy, const ns3d_cplx* sz, ns3d_cplx* dx,
+               ns3d_cplx* dy, ns3d_cplx* dz) {
+    if (!c || !sx || !sy || !sz || !dx || !dy || !dz) return NS3D_ERR_ARG;
+    PtrArr<const zd, 3> s{{(const zd*)sx, (const zd*)sy, (const zd*)sz}};
+    PtrArr<zd, 3> d{{(zd*)dx, (zd*)dy, (zd*)dz}};
+    hipLaunchKernelGGL(k_copy3, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, s, d, c->nc);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+
+int ns3d_axpby3(ns3d_ctx* c, const ns3d_cplx* ax, const ns3d_cplx* ay, const ns3d_cplx* az, double wa,
+                const ns3d_cplx* bx, const ns3d_cplx* by, const ns3d_cplx* bz, double wb, ns3d_cplx* dx,
+                ns3d_cplx* dy, ns3d_cplx* dz) {
+    if (!c || !ax || !ay || !az || !bx || !by || !bz || !dx || !dy || !dz) return NS3D_ERR_ARG;
+    PtrArr<const zd, 3> a{{(const zd*)ax, (const zd*)ay, (const zd*)az}};
+    PtrArr<const zd, 3> b{{(const zd*)bx, (const zd*)by, (const zd*)bz}};
+    PtrArr<zd, 3> d{{(zd*)dx, (zd*)dy, (zd*)dz}};
+    hipLaunchKernelGGL(k_axpby3, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, a, wa, b, wb, d, c->nc);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+
+// ---- a2 / a1
+int ns3d_stage(ns3d_ctx* c, ns3d_cplx* ux, ns3d_cplx* uy, ns3d_cplx* uz, const ns3d_cplx* fx, const ns3d_cplx* fy,
+               const ns3d_cplx* fz) {
+    if (!c || !ux || !uy || !uz) return NS3D_ERR_ARG;
+    const zd* w[3] = {(const zd*)ux, (const zd*)uy, (const zd*)uz};
+    zd* o[3] = {(zd*)ux, (zd*)uy, (zd*)uz};
+    const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
+    return stage_into(c, w, nullptr, 0.0, 1.0, o, f);
+}
+
+int ns3d_rhs(ns3d_ctx* c, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz, const ns3d_cplx* fx,
+             const ns3d_cplx* fy, const ns3d_cplx* fz, ns3d_cplx* Rx, ns3d_cplx* Ry, ns3d_cplx* Rz) {
+    if (!c || !ux || !uy || !uz || !Rx || !Ry || !Rz) return NS3D_ERR_ARG;
+    const zd* w[3] = {(const zd*)ux, (const zd*)uy, (const zd*)uz};
+    EpiArgs ea = make_epi(c);
+    const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
+    zd* R[3] = {(zd*)Rx, (zd*)Ry, (zd*)Rz};
+    for (int b = 0; b < 3; b++) {
+        ea.w[b] = (zd*)w[b];
+        ea.f[b] = f[b];
+        ea.out[b] = R[b];
+    }
+    return fused<EPI_RHS>(c, w, 1, ea);
+}
+
+int ns3d_rk3_ssp(ns3d_ctx* c, ns3d_cplx* ux, ns3d_cplx* uy, ns3d_cplx* uz, const ns3d_cplx* fx, const ns3d_cplx* fy,
+                 const ns3d_cplx* fz) {
+    return ns3d_rk3_ssp_steps(c, 1, ux, uy, uz, fx, fy, fz);
+}
+
+int ns3d_rk3_ssp_steps(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns3d_cplx* uz, const ns3d_cplx* fx,
+                       const ns3d_cplx* fy, const ns3d_cplx* fz) {
+    if (!c || !ux || !uy || !uz || nsteps < 0) return NS3D_ERR_ARG;
+    zd* u[3] = {(zd*)ux, (zd*)uy, (zd*)uz};
+    const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
+    for (int i = 0; i < nsteps; i++) {
+        int st = rk3_one(c, u, f);
+        if (st) return st;
+    }
+    return NS3D_OK;
+}
+
+// ---- a8
+int ns3d_divergence(ns3d_ctx* c, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz, ns3d_cplx* div_hat) {
+    if (!c || !ux || !uy || !uz || !div_hat) return NS3D_ERR_ARG;
+    PtrArr<const zd, 3> u{{(const zd*)ux, (const zd*)uy, (const zd*)uz}};
+    hipLaunchKernelGGL(k_divergence, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, u, (zd*)div_hat, c->nc, c->Ny,
+                       c->Mz, c->tab, c->dt);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+
+int ns3d_divergence_physical(ns3d_ctx* c, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz,
+                             ns3d_cplx* div_hat, double* div) {
+    int st = ns3d_divergence(c, ux, uy, uz, div_hat);
+    if (st) return st;
+    if (!div) return NS3D_ERR_ARG;
+    return c2r_any(c, (const zd*)div_hat, div, true);
+}
+
+// ---- a9
+int ns3d_weno_advect(ns3d_ctx* c, int scheme, const double* ux, const double* uy, const double* uz, const double* x,
+                     double* out) {
+    if (!c || !ux || !uy || !uz || !x || !out) return NS3D_ERR_ARG;
+    Grid3 G{c->Nx, c->Ny, c->Nz};
+    hipLaunchKernelGGL(k_weno, dim3(nblk(c->nr, 256)), dim3(256), 0, c->stream, ux, uy, uz, x, out, G, c->Lx / c->Nx,
+                       c->Ly / c->Ny, c->Lz / c->Nz, scheme, (c->flags & NS3D_FLAG_WENO_ZFIX) ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+
+int ns3d_weno_advection(ns3d_ctx* c, int scheme, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz,
+                        ns3d_cplx* Qx, ns3d_cplx* Qy, ns3d_cplx* Qz) {
+    if (!c || !ux || !uy || !uz || !Qx || !Qy || !Qz) return NS3D_ERR_ARG;
+    int st = ensure_real_scratch(c);
+    if (st) return st;
+    double* r = (double*)c->real_scratch;
+    const zd* u[3] = {(const zd*)ux, (const zd*)uy, (const zd*)uz};
+    zd* Q[3] = {(zd*)Qx, (zd*)Qy, (zd*)Qz};
+    for (int b = 0; b < 3; b++)
+        if ((st = c2r_any(c, u[b], r + b * c->nr, true))) return st;
+    for (int b = 0; b < 3; b++) {
+        if ((st = ns3d_weno_advect(c, scheme, r, r + c->nr, r + 2 * c->nr, r + b * c->nr, r + 3 * c->nr))) return st;
+        if ((st = r2c_any(c, r + 3 * c->nr, Q[b]))) return st;
+    }
+    return NS3D_OK;
+}
+
+// ---- a10
+int ns3d_rk3_ssp_uv_rhs(ns3d_ctx* c, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz, ns3d_cplx* vx,
+                        ns3d_cplx* vy, ns3d_cplx* vz) {
+    if (!c || !ux || !uy || !uz || !vx || !vy || !vz) return NS3D_ERR_ARG;
+    const zd* in[6] = {(const zd*)ux, (const zd*)uy, (const zd*)uz, (const zd*)vx, (const zd*)vy, (const zd*)vz};
+    EpiArgs ea = make_epi(c);
+    ea.w[0] = (zd*)vx;
+    ea.w[1] = (zd*)vy;
+    ea.w[2] = (zd*)vz;
+    return fused<EPI_UV_RHS>(c, in, 2, ea);
+}
+
+int ns3d_stage_uv(ns3d_ctx* c, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz, ns3d_cplx* vx,
+                  ns3d_cplx* vy, ns3d_cplx* vz) {
+    if (!c || !ux || !uy || !uz || !vx || !vy || !vz) return NS3D_ERR_ARG;
+    const zd* in[6] = {(const zd*)ux, (const zd*)uy, (const zd*)uz, (const zd*)vx, (const zd*)vy, (const zd*)vz};
+    EpiArgs ea = make_epi(c);
+    ea.w[0] = (zd*)vx;
+    ea.w[1] = (zd*)vy;
+    ea.w[2] = (zd*)vz;
+    return fused<EPI_UV>(c, in, 2, ea);
+}
+
+int ns3d_stage_iuv_rhs(ns3d_ctx* c, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz, ns3d_cplx* vx,
+                       ns3d_cplx* vy, ns3d_cplx* vz) {
+    if (!c || !ux || !uy || !uz || !vx || !vy || !vz) return NS3D_ERR_ARG;
+    const zd* in[6] = {(const zd*)ux, (const zd*)uy, (const zd*)uz, (const zd*)vx, (const zd*)vy, (const zd*)vz};
+    EpiArgs ea = make_epi(c);
+    ea.w[0] = (zd*)vx;
+    ea.w[1] = (zd*)vy;
+    ea.w[2] = (zd*)vz;
+    return fused<EPI_IUV>(c, in, 2, ea);
+}
+
+int ns3d_rk3_ssp_uv(ns3d_ctx* c, ns3d_cplx* ux, ns3d_cplx* uy, ns3d_cplx* uz, ns3d_cplx* vx, ns3d_cplx* vy,
+                    ns3d_cplx* vz, const ns3d_cplx* fx, const ns3d_cplx* fy, const ns3d_cplx* fz) {
+    if (!c || !ux || !uy || !uz || !vx || !vy || !vz) return NS3D_ERR_ARG;
+    int st = ensure_fp64(c, &c->extra, 6);
+    if (st) return st;
+    zd* v1[3] = {c->extra, c->extra + c->nc, c->extra + 2 * c->nc};
+    // RK3_SSP_UV (Source/RK_time_step.cu:342-355)
+    if ((st = ns3d_copy3(c, vx, vy, vz, (ns3d_cplx*)v1[0], (ns3d_cplx*)v1[1], (ns3d_cplx*)v1[2]))) return st;
+    if ((st = ns3d_stage_uv(c, ux, uy, uz, (ns3d_cplx*)v1[0], (ns3d_cplx*)v1[1], (ns3d_cplx*)v1[2]))) return st;
+    if ((st = ns3d_rk3_ssp(c, ux, uy, uz, fx, fy, fz))) return st;
+    if ((st = ns3d_stage_uv(c, ux, uy, uz, (ns3d_cplx*)v1[0], (ns3d_cplx*)v1[1], (ns3d_cplx*)v1[2]))) return st;
+    return ns3d_axpby3(c, vx, vy, vz, 0.5, (ns3d_cplx*)v1[0], (ns3d_cplx*)v1[1], (ns3d_cplx*)v1[2], 0.5, vx, vy, vz);
+}
+
+int ns3d_unpack_reduced(ns3d_ctx* c, const double* vec, ns3d_cplx* vx, ns3d_cplx* vy, ns3d_cplx* vz) {
+    if (!c || !vec || !vx || !vy || !vz) return NS3D_ERR_ARG;
+    PtrArr<zd, 3> v{{(zd*)vx, (zd*)vy, (zd*)vz}};
+    hipLaunchKernelGGL(k_unpack_reduced, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, vec, v, c->nc);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+int ns3d_pack_reduced(ns3d_ctx* c, const ns3d_cplx* vx, const ns3d_cplx* vy, const ns3d_cplx* vz, double* vec) {
+    if (!c || !vec || !vx || !vy || !vz) return NS3D_ERR_ARG;
+    PtrArr<const zd, 3> v{{(const zd*)vx, (const zd*)vy, (const zd*)vz}};
+    hipLaunchKernelGGL(k_pack_reduced, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, v, vec, c->nc);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+int ns3d_unpack_full(ns3d_ctx* c, const double* vec, ns3d_cplx* vx, ns3d_cplx* vy, ns3d_cplx* vz) {
+    if (!c || !vec || !vx || !vy || !vz) return NS3D_ERR_ARG;
+    PtrArr<zd, 3> v{{(zd*)vx, (zd*)vy, (zd*)vz}};
+    hipLaunchKernelGGL(k_unpack_full, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, vec, v, c->nc);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+int ns3d_pack_full(ns3d_ctx* c, const ns3d_cplx* vx, const ns3d_cplx* vy, const ns3d_cplx* vz, double* vec) {
+    if (!c || !vec || !vx || !vy || !vz) return NS3D_ERR_ARG;
+    PtrArr<const zd, 3> v{{(const zd*)vx, (const zd*)vy, (const zd*)vz}};
+    hipLaunchKernelGGL(k_pack_full, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, v, vec, c->nc);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+
+int ns3d_set_base_state(ns3d_ctx* c, ns3d_cplx* ux, ns3d_cplx* uy, ns3d_cplx* uz, const ns3d_cplx* fx,
+                        const ns3d_cplx* fy, const ns3d_cplx* fz, int timesteps_period, double tau) {
+    if (!c || !ux || !uy || !uz || timesteps_period < 0) return NS3D_ERR_ARG;
+    c->base[0] = (zd*)ux;
+    c->base[1] = (zd*)uy;
+    c->base[2] = (zd*)uz;
+    c->bforce[0] = (const zd*)fx;
+    c->bforce[1] = (const zd*)fy;
+    c->bforce[2] = (const zd*)fz;
+    c->timesteps_period = timesteps_period;
+    c->tau = tau;
+    return ensure_fp64(c, &c->kry, 3);
+}
+
+}  // extern "C"
+
+namespace {
+int jv_reduced_impl(ns3d_ctx* c, const double* in, double* out, bool newton) {
+    if (!c || !in || !out || !c->base[0] || !c->kry) return NS3D_ERR_ARG;
+    ns3d_cplx* v[3] = {(ns3d_cplx*)c->kry, (ns3d_cplx*)(c->kry + c->nc), (ns3d_cplx*)(c->kry + 2 * c->nc)};
+    ns3d_cplx* u[3] = {(ns3d_cplx*)c->base[0], (ns3d_cplx*)c->base[1], (ns3d_cplx*)c->base[2]};
+    int st;
+    // NSCallMatrixVector_reduced (Source/NS3D_periodic.cu:340-361)
+    if ((st = ns3d_unpack_reduced(c, in, v[0], v[1], v[2]))) return st;
+    if ((st = ns3d_rk3_ssp_uv_rhs(c, u[0], u[1], u[2], v[0], v[1], v[2]))) return st;
+    if (!newton) return ns3d_pack_reduced(c, v[0], v[1], v[2], out);
+    // ..._Newton_Globalization (:364-389): out = pack(J v) - v/tau
+    PtrArr<const zd, 3> vv{{(const zd*)v[0], (const zd*)v[1], (const zd*)v[2]}};
+    hipLaunchKernelGGL(k_pack_reduced_shifted, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, vv, in, 1.0 / c->tau,
+                       out, c->nc);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+}  // namespace
+
+extern "C" {
+
+void ns3d_jv_reduced(void* ctx, void* vec_in, void* vec_out) {
+    ns3d_ctx* c = (ns3d_ctx*)ctx;
+    set_err(c, jv_reduced_impl(c, (const double*)vec_in, (double*)vec_out, false));
+}
+
+void ns3d_jv_reduced_newton(void* ctx, void* vec_in, void* vec_out) {
+    ns3d_ctx* c = (ns3d_ctx*)ctx;
+    set_err(c, jv_reduced_impl(c, (const double*)vec_in, (double*)vec_out, true));
+}
+
+void ns3d_jv_exponential(void* ctx, void* vec_in, void* vec_out) {
+    ns3d_ctx* c = (ns3d_ctx*)ctx;
+    if (!c || !vec_in || !vec_out || !c->base[0] || !c->kry) {
+        set_err(c, NS3D_ERR_ARG);
+        return;
+    }
+    ns3d_cplx* v[3] = {(ns3d_cplx*)c->kry, (ns3d_cplx*)(c->kry + c->nc), (ns3d_cplx*)(c->kry + 2 * c->nc)};
+    ns3d_cplx* u[3] = {(ns3d_cplx*)c->base[0], (ns3d_cplx*)c->base[1], (ns3d_cplx*)c->base[2]};
+    const ns3d_cplx* f[3] = {(const ns3d_cplx*)c->bforce[0], (const ns3d_cplx*)c->bforce[1], (const ns3d_cplx*)c->bforce[2]};
+    // NSCallMatrixVector_exponential (Source/NS3D_periodic.cu:444-470)
+    int st = ns3d_unpack_full(c, (const double*)vec_in, v[0], v[1], v[2]);
+    for (int tt = 0; !st && tt < c->timesteps_period; tt++)
+        st = ns3d_rk3_ssp_uv(c, u[0], u[1], u[2], v[0], v[1], v[2], f[0], f[1], f[2]);
+    if (!st) st = ns3d_pack_full(c, v[0], v[1], v[2], (double*)vec_out);
+    set_err(c, st);
+}
+
+// ---- a11 / a12
+int ns3d_reduce_minmax(ns3d_ctx* c, long long n, const double* in, double* out2) {
+    if (!c || !in || !out2 || n < 0) return NS3D_ERR_ARG;
+    RedArgs r = red_args(c, in, nullptr, nullptr);
+    return reduce<1, SRC_ARRAY>(c, r, (long)n, out2);
+}
+int ns3d_reduce_sum(ns3d_ctx* c, long long n, const double* in, double* out) {
+    if (!c || !in || !out || n < 0) return NS3D_ERR_ARG;
+    RedArgs r = red_args(c, in, nullptr, nullptr);
+    return reduce<0, SRC_ARRAY>(c, r, (long)n, out);
+}
+int ns3d_kinetic_energy(ns3d_ctx* c, const double* ux, const double* uy, const double* uz, double* out) {
+    if (!c || !ux || !uy || !uz || !out) return NS3D_ERR_ARG;
+    RedArgs r = red_args(c, ux, uy, uz);
+    return reduce<0, SRC_ENERGY>(c, r, c->nr, out);
+}
+int ns3d_dissipation(ns3d_ctx* c, const double* ux, const double* uy, const double* uz, double* out) {
+    if (!c || !ux || !uy || !uz || !out) return NS3D_ERR_ARG;
+    RedArgs r = red_args(c, ux, uy, uz);
+    return reduce<0, SRC_DISSIPATION>(c, r, c->nr, out);
+}
+int ns3d_cfl_dt(ns3d_ctx* c, double cfl, const double* ux, const double* uy, const double* uz, double* dt) {
+    if (!c || !ux || !uy || !uz || !dt) return NS3D_ERR_ARG;
+    RedArgs r = red_args(c, ux, uy, uz);
+    r.p0 = cfl;
+    double mm[2];
+    int st = reduce<1, SRC_CFL>(c, r, c->nr, mm);
+    if (!st) *dt = mm[0];
+    return st;
+}
+int ns3d_shapiro_error(ns3d_ctx* c, double t, const double* ux, const double* uy, const double* uz, double* err) {
+    if (!c || !ux || !uy || !uz || !err) return NS3D_ERR_ARG;
+    RedArgs r = red_args(c, ux, uy, uz);
+    r.p0 = t;
+    double mm[2];
+    int st = reduce<1, SRC_SHAPIRO>(c, r, c->nr, mm);
+    if (!st) *err = mm[1];
+    return st;
+}
+
+}  // extern "C"

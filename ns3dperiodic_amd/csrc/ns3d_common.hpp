@@ -1,0 +1,79 @@
+// ns3d_common.hpp -- shared device types and spectral-table helpers for the
+// gfx950 NS3DPeriodic hot path.
+//
+// Layout (reference Source/Macros.h:11,15): spectral fields are [Nx][Ny][Mz]
+// interleaved complex, Mz = Nz/2+1, z fastest; physical fields [Nx][Ny][Nz].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ns3d {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cplx {
+    T x, y;
+};
+
+template <typename T> __device__ __forceinline__ cplx<T> mk(T a, T b) { cplx<T> r; r.x = a; r.y = b; return r; }
+template <typename T> __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T> __device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+template <typename T> __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) { return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <typename T> __device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
+template <typename T> __device__ __forceinline__ cplx<T> conjg(cplx<T> a) { return mk<T>(a.x, -a.y); }
+// multiply by (sgn * i)
+template <int SGN, typename T> __device__ __forceinline__ cplx<T> mul_i(cplx<T> a) {
+    return SGN > 0 ? mk<T>(-a.y, a.x) : mk<T>(a.y, -a.x);
+}
+template <typename TO, typename TI> __device__ __forceinline__ cplx<TO> cvt(cplx<TI> a) { return mk<TO>((TO)a.x, (TO)a.y); }
+
+// Per-context spectral constants.  Per-axis tables are built on the host with
+// the same floating-point expressions as the reference's host builders
+// (Source/math_support.cu:374-519) so that on-the-fly recomposition on the
+// device reproduces the reference tables exactly.
+struct SpecTables {
+    const double* kx;       // [Nx]  2*pi/Lx*m          (build_Nabla_Wavenumbers :393-414)
+    const double* ky;       // [Ny]
+    const double* kz;       // [Mz]
+    const double* sx;       // [Nx]  kx*kx   (din_diffusion = sx+sy+sz, :374-390)
+    const double* sy;       // [Ny]
+    const double* sz;       // [Mz]
+    const double* px;       // [Nx]  k1*k1*m*m (projection denominators, :418-466)
+    const double* py;       // [Ny]
+    const double* pz;       // [Mz]
+    const double* fm;       // [Nx]  (double)m       (AM off-diagonal numerators)
+    const double* fn;       // [Ny]  (double)n
+    const double* fq;       // [Mz]  (double)q
+    const double* mx;       // [Nx]  kx^2/kxMax^2  (build_mask_matrix :471-519, with Nz := Mz)
+    const double* my;       // [Ny]
+    const double* mz;       // [Mz]
+    double k1k2, k1k3, k2k3;  // products of the base wavenumbers 2*pi/L
+    double mask_smax;       // largest s with sqrt(s) < 2/3 (host libm sqrt)
+};
+
+// 2/3 mask of the reference (Q2): sqrt(mx+my+mz) < 2/3 and M(0)=0.
+__device__ __forceinline__ bool mask_keep(const SpecTables& t, int j, int k, int l) {
+    const double s = (t.mx[j] + t.my[k]) + t.mz[l];
+    return (s <= t.mask_smax) && ((j | k | l) != 0);
+}
+
+// Reference AM_ab and din_diffusion at (j,k,l).
+struct ProjCoef {
+    double a11, a22, a33, a12, a13, a23, din;
+};
+__device__ __forceinline__ ProjCoef proj_coef(const SpecTables& t, int j, int k, int l) {
+    ProjCoef c;
+    double den = (t.px[j] + t.py[k]) + t.pz[l];
+    const bool zero = ((j | k | l) == 0);
+    if (zero) den = 1.0;
+    c.a11 = t.px[j] / den;
+    c.a22 = t.py[k] / den;
+    c.a33 = t.pz[l] / den;
+    c.a12 = ((t.k1k2 * t.fm[j]) * t.fn[k]) / den;
+    c.a13 = ((t.k1k3 * t.fm[j]) * t.fq[l]) / den;
+    c.a23 = ((t.k2k3 * t.fn[k]) * t.fq[l]) / den;
+    if (zero) { c.a11 = c.a22 = c.a33 = c.a12 = c.a13 = c.a23 = 0.0; }
+    c.din = (t.sx[j] + t.sy[k]) + t.sz[l];
+    return c;
+}
+
+}  // namespace ns3d

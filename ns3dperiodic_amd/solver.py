@@ -1,0 +1,274 @@
+"""Host-side mirror of NS3DPeriodic's hot-path operators over the HIP C ABI.
+
+Method names follow the reference functions they replace (RK_time_step.h,
+advection_2_3.h, advection_WENO.h, diffusion.h, divergence.h,
+min_max_reduction.h, math_support.h, the J.v callbacks of NS3D_periodic.cu).
+Fields are torch tensors on the GPU in the reference layout:
+  spectral  complex128 [Nx, Ny, Nz//2+1]  (interleaved, = cufftDoubleComplex)
+  physical  float64    [Nx, Ny, Nz]
+torch provides device memory and the stream; all arithmetic runs in the
+hand-written gfx950 kernels of libns3d_hip.so.
+"""
+import ctypes
+import math
+
+import torch
+
+from . import _lib
+from ._lib import check
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("ns3d fields must live on the GPU")
+    if not t.is_contiguous():
+        raise ValueError("ns3d fields must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class NS3D:
+    """One grid/box configuration; owns an ns3d_ctx (struct_NS3D_RK3_Call analogue)."""
+
+    def __init__(self, Nx, Ny=None, Nz=None, Lx=2 * math.pi, Ly=2 * math.pi, Lz=2 * math.pi,
+                 Re=100.0, dt=1e-3, precision="fp64", ipq=True, weno_zfix=False, device=0, stream=None):
+        self.lib = _lib.load()
+        Ny = Nx if Ny is None else Ny
+        Nz = Nx if Nz is None else Nz
+        self.Nx, self.Ny, self.Nz, self.Mz = Nx, Ny, Nz, Nz // 2 + 1
+        self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
+        self.Re, self.dt = Re, dt
+        self.device = torch.device("cuda", device)
+        flags = (0 if ipq else _lib.NS3D_FLAG_NO_IPQ) | (_lib.NS3D_FLAG_WENO_ZFIX if weno_zfix else 0)
+        prec = {"fp64": _lib.NS3D_FP64, "fp32": _lib.NS3D_FP32_BULK}[precision]
+        self.precision = precision
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        self.stream = stream
+        cfg = _lib.Config(Nx, Ny, Nz, Lx, Ly, Lz, Re, dt, prec, flags, ctypes.c_void_p(stream.cuda_stream), device)
+        h = ctypes.c_void_p()
+        check(self.lib.ns3d_create(ctypes.byref(cfg), ctypes.byref(h)), "ns3d_create")
+        self.ctx = h
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.ns3d_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ helpers
+    @property
+    def cshape(self):
+        return (self.Nx, self.Ny, self.Mz)
+
+    @property
+    def rshape(self):
+        return (self.Nx, self.Ny, self.Nz)
+
+    @property
+    def n_complex(self):
+        return self.Nx * self.Ny * self.Mz
+
+    def cfield(self, n=1):
+        z = [torch.zeros(self.cshape, dtype=torch.complex128, device=self.device) for _ in range(n)]
+        return z if n > 1 else z[0]
+
+    def rfield(self, n=1):
+        z = [torch.zeros(self.rshape, dtype=torch.float64, device=self.device) for _ in range(n)]
+        return z if n > 1 else z[0]
+
+    def _c3(self, u):
+        assert len(u) == 3
+        for c in u:
+            assert c.dtype == torch.complex128 and tuple(c.shape) == self.cshape, (c.dtype, c.shape)
+        return [_ptr(c) for c in u]
+
+    def _r3(self, u):
+        assert len(u) == 3
+        for c in u:
+            assert c.dtype == torch.float64 and tuple(c.shape) == self.rshape, (c.dtype, c.shape)
+        return [_ptr(c) for c in u]
+
+    def _f3(self, f):
+        return [None, None, None] if f is None else self._c3(f)
+
+    def set_params(self, dt=None, Re=None):
+        self.dt = self.dt if dt is None else dt
+        self.Re = self.Re if Re is None else Re
+        check(self.lib.ns3d_set_params(self.ctx, self.dt, self.Re), "ns3d_set_params")
+
+    def synchronize(self):
+        check(self.lib.ns3d_synchronize(self.ctx), "ns3d_synchronize")
+
+    # ------------------------------------------------------------ a4 FFT layer
+    def FFTN(self, x):
+        """FFTN_Device: D2Z, unnormalised."""
+        out = torch.empty(self.cshape, dtype=torch.complex128, device=self.device)
+        check(self.lib.ns3d_fft_r2c(self.ctx, _ptr(x.contiguous()), _ptr(out)), "ns3d_fft_r2c")
+        return out
+
+    def iFFTN(self, X, scale=True):
+        """iFFTN_Device: Z2D (scaled by 1/(NxNyNz) as with scale_double)."""
+        out = torch.empty(self.rshape, dtype=torch.float64, device=self.device)
+        fn = self.lib.ns3d_fft_c2r if scale else self.lib.ns3d_fft_c2r_unscaled
+        check(fn(self.ctx, _ptr(X.contiguous()), _ptr(out)), "ns3d_fft_c2r")
+        return out
+
+    # ------------------------------------------------------------ a5 tables
+    def tables(self):
+        t = {k: torch.empty(self.cshape, dtype=torch.float64, device=self.device) for k in
+             ("din_diffusion", "din_poisson", "AM11", "AM22", "AM33", "AM12", "AM13", "AM23", "mask")}
+        t["kx"] = torch.empty(self.Nx, dtype=torch.float64, device=self.device)
+        t["ky"] = torch.empty(self.Ny, dtype=torch.float64, device=self.device)
+        t["kz"] = torch.empty(self.Mz, dtype=torch.float64, device=self.device)
+        check(self.lib.ns3d_tables(self.ctx, *[_ptr(t[k]) for k in (
+            "kx", "ky", "kz", "din_diffusion", "din_poisson", "AM11", "AM22", "AM33", "AM12", "AM13", "AM23",
+            "mask")]), "ns3d_tables")
+        return t
+
+    # ------------------------------------------------------------ a3 nonlinear
+    def calculate_convolution_2p3(self, u):
+        Q = self.cfield(3)
+        check(self.lib.ns3d_nonlinear(self.ctx, *self._c3(u), *self._c3(Q)), "ns3d_nonlinear")
+        return Q
+
+    def calculate_convolution_2p3_UV(self, u, v):
+        Q = self.cfield(3)
+        check(self.lib.ns3d_bilinear(self.ctx, *self._c3(u), *self._c3(v), *self._c3(Q)), "ns3d_bilinear")
+        return Q
+
+    # ------------------------------------------------------------ a6 update family (in place on u)
+    def project_update(self, mode, u, Q, f=None):
+        R = self.cfield(3) if mode == _lib.UPD_RHS else [None, None, None]
+        Rp = self._c3(R) if mode == _lib.UPD_RHS else R
+        check(self.lib.ns3d_project_update(self.ctx, mode, *self._c3(u), *self._f3(f), *self._c3(Q), *Rp),
+              "ns3d_project_update")
+        return R if mode == _lib.UPD_RHS else u
+
+    # ------------------------------------------------------------ a7
+    def copy_arrays(self, src, dst):
+        check(self.lib.ns3d_copy3(self.ctx, *self._c3(src), *self._c3(dst)), "ns3d_copy3")
+
+    def single_RK_step(self, a, wa, b, wb, dst):
+        check(self.lib.ns3d_axpby3(self.ctx, *self._c3(a), wa, *self._c3(b), wb, *self._c3(dst)), "ns3d_axpby3")
+
+    # ------------------------------------------------------------ a2 / a1 (in place on u)
+    def single_time_step(self, u, f=None):
+        check(self.lib.ns3d_stage(self.ctx, *self._c3(u), *self._f3(f)), "ns3d_stage")
+        return u
+
+    def return_RHS(self, u, f=None):
+        R = self.cfield(3)
+        check(self.lib.ns3d_rhs(self.ctx, *self._c3(u), *self._f3(f), *self._c3(R)), "ns3d_rhs")
+        return R
+
+    def RK3_SSP(self, u, f=None, steps=1):
+        check(self.lib.ns3d_rk3_ssp_steps(self.ctx, steps, *self._c3(u), *self._f3(f)), "ns3d_rk3_ssp_steps")
+        return u
+
+    # ------------------------------------------------------------ a8
+    def divergence(self, u):
+        d = self.cfield(1)
+        check(self.lib.ns3d_divergence(self.ctx, *self._c3(u), _ptr(d)), "ns3d_divergence")
+        return d
+
+    def divergence_physical(self, u):
+        d = self.cfield(1)
+        r = self.rfield(1)
+        check(self.lib.ns3d_divergence_physical(self.ctx, *self._c3(u), _ptr(d), _ptr(r)), "ns3d_divergence_physical")
+        return d, r
+
+    # ------------------------------------------------------------ a9
+    def weno_advect(self, scheme, u, x):
+        out = self.rfield(1)
+        check(self.lib.ns3d_weno_advect(self.ctx, scheme, *self._r3(u), _ptr(x), _ptr(out)), "ns3d_weno_advect")
+        return out
+
+    def WenoAdvection(self, scheme, u_hat):
+        Q = self.cfield(3)
+        check(self.lib.ns3d_weno_advection(self.ctx, scheme, *self._c3(u_hat), *self._c3(Q)), "ns3d_weno_advection")
+        return Q
+
+    # ------------------------------------------------------------ a10 (in place on v; RK3_SSP_UV also on u)
+    def RK3_SSP_UV_RHS(self, u, v):
+        check(self.lib.ns3d_rk3_ssp_uv_rhs(self.ctx, *self._c3(u), *self._c3(v)), "ns3d_rk3_ssp_uv_rhs")
+        return v
+
+    def single_time_step_UV(self, u, v):
+        check(self.lib.ns3d_stage_uv(self.ctx, *self._c3(u), *self._c3(v)), "ns3d_stage_uv")
+        return v
+
+    def single_time_step_iUV_RHS(self, u, v):
+        check(self.lib.ns3d_stage_iuv_rhs(self.ctx, *self._c3(u), *self._c3(v)), "ns3d_stage_iuv_rhs")
+        return v
+
+    def RK3_SSP_UV(self, u, v, f=None):
+        check(self.lib.ns3d_rk3_ssp_uv(self.ctx, *self._c3(u), *self._c3(v), *self._f3(f)), "ns3d_rk3_ssp_uv")
+        return u, v
+
+    def vector_length_reduced(self):
+        return 3 * (self.n_complex - 1)
+
+    def unpack_reduced(self, vec):
+        v = self.cfield(3)
+        check(self.lib.ns3d_unpack_reduced(self.ctx, _ptr(vec), *self._c3(v)), "ns3d_unpack_reduced")
+        return v
+
+    def pack_reduced(self, v):
+        vec = torch.empty(self.vector_length_reduced(), dtype=torch.float64, device=self.device)
+        check(self.lib.ns3d_pack_reduced(self.ctx, *self._c3(v), _ptr(vec)), "ns3d_pack_reduced")
+        return vec
+
+    def set_base_state(self, u, f=None, timesteps_period=1, tau=1.0):
+        self._base = (u, f)  # keep the tensors alive while registered
+        check(self.lib.ns3d_set_base_state(self.ctx, *self._c3(u), *self._f3(f), timesteps_period, tau),
+              "ns3d_set_base_state")
+
+    def jv_reduced(self, vec_in, newton=False):
+        out = torch.empty_like(vec_in)
+        fn = self.lib.ns3d_jv_reduced_newton if newton else self.lib.ns3d_jv_reduced
+        fn(self.ctx, _ptr(vec_in), _ptr(out))
+        check(self.lib.ns3d_last_error(self.ctx), "ns3d_jv_reduced")
+        return out
+
+    def jv_exponential(self, vec_in, vec_out):
+        self.lib.ns3d_jv_exponential(self.ctx, _ptr(vec_in), _ptr(vec_out))
+        check(self.lib.ns3d_last_error(self.ctx), "ns3d_jv_exponential")
+        return vec_out
+
+    # ------------------------------------------------------------ a11 / a12 (host results)
+    def compute_reduction(self, x):
+        out = (ctypes.c_double * 2)()
+        check(self.lib.ns3d_reduce_minmax(self.ctx, x.numel(), _ptr(x), out), "ns3d_reduce_minmax")
+        return out[0], out[1]
+
+    def reduction_sum(self, x):
+        out = ctypes.c_double()
+        check(self.lib.ns3d_reduce_sum(self.ctx, x.numel(), _ptr(x), ctypes.byref(out)), "ns3d_reduce_sum")
+        return out.value
+
+    def get_kinetic_energy(self, u):
+        out = ctypes.c_double()
+        check(self.lib.ns3d_kinetic_energy(self.ctx, *self._r3(u), ctypes.byref(out)), "ns3d_kinetic_energy")
+        return out.value
+
+    def get_dissipation(self, u):
+        out = ctypes.c_double()
+        check(self.lib.ns3d_dissipation(self.ctx, *self._r3(u), ctypes.byref(out)), "ns3d_dissipation")
+        return out.value
+
+    def calc_dt(self, cfl, u):
+        out = ctypes.c_double()
+        check(self.lib.ns3d_cfl_dt(self.ctx, cfl, *self._r3(u), ctypes.byref(out)), "ns3d_cfl_dt")
+        return out.value
+
+    def Shapiro_test_case(self, t, u):
+        out = ctypes.c_double()
+        check(self.lib.ns3d_shapiro_error(self.ctx, t, *self._r3(u), ctypes.byref(out)), "ns3d_shapiro_error")
+        return out.value
