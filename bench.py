@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""RK3 steps/s of the fused gfx950 pseudo-spectral step (NS3DPeriodic hot path).
+
+Workload: decaying turbulence at 256^3 on [0,2pi]^3 (nu = 2e-3, i.e. Re = 500),
+seeded solenoidal field with E(k) ~ k^4 exp(-2 (k/4)^2), energy 0.5, reference
+mode (2/3 mask, IPQ rule).  A step is one RK3_SSP call on state resident in HBM.
+
+Prints ONE JSON line.  `value` is steps/s of the primary precision (default
+FP64, the parity-pinned path); the other precision is reported beside it.
+`roofline` is for the dominant kernel: algorithmic bytes per launch / average
+launch duration, both measured here with HIP events on the library's stream.
+`cpu_baseline` times the CPU oracle (FP64 restatement, OpenMP) on a bounded
+sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+PASS_NAMES = ["x_inverse", "y_inverse", "z_product", "y_forward", "x_forward_epilogue"]
+PASS_KERNELS = ["k_x_inv_grad", "k_y_inv_grad", "k_z_product", "k_axis_c2c", "k_x_fwd_epi"]
+
+
+def pass_bytes(nc, bulk_bytes):
+    """Algorithmic HBM bytes per launch of each pass of one RK3 stage.
+
+    Field traffic of the five-pass dataflow (S = one spectral field of nc
+    modes): P1 reads 3 state fields and writes 6; P2 reads 6, writes 9; P3
+    reads 9, writes 3; P4 reads 3, writes 3; P5 reads 3 bulk + 3 state (+3
+    u^n in stages 2-3) and writes 3 state.  State fields are FP64 complex
+    (16 B); bulk fields use the bulk precision.  With FP64 bulk the RK3 step
+    totals 159 S, the algorithmic byte count of the path.
+    """
+    s, b = 16 * nc, bulk_bytes * nc
+    p5_avg = 3 * b + 3 * s + 3 * s + (2.0 / 3.0) * 3 * s
+    return [3 * s + 6 * b, 15 * b, 12 * b, 6 * b, p5_avg]
+
+
+def step_bytes(nc, bulk_bytes):
+    pb = pass_bytes(nc, bulk_bytes)
+    return 3 * sum(pb)
+
+
+def run_gpu(N, precision, steps, warmup, u0, Re, dt):
+    import torch
+    from ns3dperiodic_amd import NS3D
+
+    g = NS3D(N, Re=Re, dt=dt, precision=precision)
+    u = [torch.from_numpy(c).to("cuda:0") for c in u0]
+    g.RK3_SSP(u, None, steps=warmup)
+    torch.cuda.synchronize()
+    lib = g.lib
+    import ctypes
+    lib.ns3d_profile_enable(g.ctx, 1)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    g.RK3_SSP(u, None, steps=steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = (ctypes.c_double * 5)()
+    calls = (ctypes.c_longlong * 5)()
+    lib.ns3d_profile_read(g.ctx, ms, calls)
+    lib.ns3d_profile_enable(g.ctx, 0)
+    total_ms = e0.elapsed_time(e1)
+    finite = bool(torch.isfinite(u[0]).all().item())
+    g.close()
+    bulk = 16 if precision == "fp64" else 8
+    nc = N * N * (N // 2 + 1)
+    pb = pass_bytes(nc, bulk)
+    passes = []
+    for i in range(5):
+        avg = ms[i] / calls[i] if calls[i] else float("nan")
+        gbs = pb[i] / (avg * 1e-3) / 1e9 if calls[i] else float("nan")
+        passes.append({"pass": PASS_NAMES[i], "kernel": PASS_KERNELS[i], "calls": calls[i], "avg_ms": avg,
+                       "total_ms": ms[i], "alg_bytes": pb[i], "GBps": gbs})
+    ms_step = total_ms / steps
+    sb = step_bytes(nc, bulk)
+    return {
+        "precision": precision,
+        "steps_per_s": steps / (total_ms * 1e-3),
+        "ms_per_step": ms_step,
+        "wall_s": wall,
+        "step_alg_bytes": sb,
+        "step_GBps": sb / (ms_step * 1e-3) / 1e9,
+        "passes": passes,
+        "finite": finite,
+    }
+
+
+def cpu_baseline(N, u0, Re, dt, steps, threads):
+    from oracle.oracle import Oracle
+    o = Oracle(N, Re=Re, dt=dt, nthreads=threads)
+    zero = [np.zeros(o.cshape, np.complex128)] * 3
+    t0 = time.perf_counter()
+    o.rk3(u0, zero, steps=steps)
+    dt_s = time.perf_counter() - t0
+    return {"value": steps / dt_s, "unit": "RK3 steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} RK3 step(s) of the same {N}^3 FP64 workload, oracle C restatement "
+                      f"(OpenMP, own mixed-radix FFT, not FFTW), {dt_s:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--secondary", choices=["fp64", "fp32", "none"], default="fp32")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
+              "(the multi-GPU slab decomposition is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from ns3dperiodic_amd import fields
+
+    N = args.N
+    Re = 500.0                      # nu = 2e-3
+    u0 = fields.random_solenoidal(N, seed=args.seed, kind="k4exp", k0=4.0, energy=0.5)
+    umax = max(float(np.abs(np.fft.irfftn(c, s=(N, N, N), axes=(0, 1, 2))).max()) for c in u0)
+    dx = 2 * math.pi / N
+    dt = float("%.1e" % (0.2 * dx / umax))  # fixed dt = 0.2 dx / max|u0|, rounded
+    print(f"[bench] N={N} Re={Re} dt={dt:g} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
+
+    prim = run_gpu(N, args.precision, args.steps, args.warmup, u0, Re, dt)
+    print(f"[bench] {args.precision}: {prim['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+    sec = None
+    if args.secondary != "none" and args.secondary != args.precision:
+        sec = run_gpu(N, args.secondary, args.steps, args.warmup, u0, Re, dt)
+        print(f"[bench] {args.secondary}: {sec['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+
+    dom = max(prim["passes"], key=lambda p: p["total_ms"])
+    roofline = {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                "alg_bytes_per_launch": dom["alg_bytes"], "avg_launch_ms": dom["avg_ms"]}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count(), os.cpu_count(), 16)
+        cpu = cpu_baseline(N, u0, Re, dt, args.cpu_steps, threads)
+        print(f"[bench] cpu: {cpu['value']:.4f} steps/s", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "RK3 steps/s at 256^3 (FP64; FP32-bulk beside it), % of HBM peak",
+        "value": prim["steps_per_s"],
+        "unit": "RK3 steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": prim["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 bulk (f64 state)",
+        "data": "synthetic",
+        "config": {"workload": f"decaying turbulence {N}^3, nu=2e-3, k0=4 spectrum, reference mode",
+                   "N": N, "Re": Re, "dt": dt, "global_batch": 1, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "step_roofline": {"alg_bytes": prim["step_alg_bytes"], "achieved": prim["step_GBps"],
+                          "frac": prim["step_GBps"] / HBM_PEAK_GBS},
+        "passes": prim["passes"],
+        "finite": prim["finite"],
+    }
+    if sec is not None:
+        out[f"{sec['precision']}_bulk" if sec["precision"] == "fp32" else sec["precision"]] = {
+            "steps_per_s": sec["steps_per_s"], "ms_per_step": sec["ms_per_step"],
+            "speedup_vs_primary": sec["steps_per_s"] / prim["steps_per_s"],
+            "step_roofline_frac": sec["step_GBps"] / HBM_PEAK_GBS,
+            "passes": sec["passes"], "finite": sec["finite"]}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

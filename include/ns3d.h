@@ -164,6 +164,13 @@ int ns3d_cfl_dt(ns3d_ctx *ctx, double cfl, const double *ux, const double *uy, c
 int ns3d_shapiro_error(ns3d_ctx *ctx, double t, const double *ux, const double *uy, const double *uz,
                        double *err_host);                                          /* Shapiro_test_case: L-inf error */
 
+/* ---- measurement: per-pass device time of the fused pipeline, recorded with
+ *      HIP events on the context stream while enabled.  ms[5]/calls[5] are
+ *      indexed 0 x-inverse, 1 y-inverse, 2 z product, 3 y-forward,
+ *      4 x-forward + epilogue.  No reference counterpart (timing hook). ---- */
+int ns3d_profile_enable(ns3d_ctx *ctx, int on);
+int ns3d_profile_read(ns3d_ctx *ctx, double *ms, long long *calls);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,8 @@ SIGNATURES = {
     "ns3d_dissipation": (I, [P, P, P, P, DP]),
     "ns3d_cfl_dt": (I, [P, D, P, P, P, DP]),
     "ns3d_shapiro_error": (I, [P, D, P, P, P, DP]),
+    "ns3d_profile_enable": (I, [P, I]),
+    "ns3d_profile_read": (I, [P, DP, ctypes.POINTER(LL)]),
 }
 
 _lib = None

@@ -83,6 +83,12 @@ struct ns3d_ctx {
     int timesteps_period = 1;
     double tau = 1.0;
     zd* kry = nullptr;  // 3 fields for J.v unpacking
+    // optional per-pass timing with HIP events on the context stream
+    bool prof_on = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_used;  // (pass id, index of start event; stop = +1)
+    hipEvent_t ev_open = nullptr;
+    int ev_open_pass = -1;
 };
 
 namespace {
@@ -90,6 +96,30 @@ namespace {
 int set_err(ns3d_ctx* c, int s) {
     if (c) c->last_error = s;
     return s;
+}
+
+// ---------------------------------------------------------------- pass timing
+// Pass ids: 0 x-inverse, 1 y-inverse, 2 z product, 3 y-forward, 4 x-forward+epilogue.
+constexpr int NPASS = 5;
+hipEvent_t prof_event(ns3d_ctx* c, size_t i) {
+    while (c->ev_pool.size() <= i) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[i];
+}
+void prof_begin(ns3d_ctx* c, int pass) {
+    if (!c->prof_on) return;
+    const size_t i = 2 * c->ev_used.size();
+    hipEvent_t a = prof_event(c, i), b = prof_event(c, i + 1);
+    if (!a || !b) return;
+    hipEventRecord(a, c->stream);
+    c->ev_used.push_back({pass, (int)i});
+}
+void prof_end(ns3d_ctx* c) {
+    if (!c->prof_on || c->ev_used.empty()) return;
+    hipEventRecord(c->ev_pool[c->ev_used.back().second + 1], c->stream);
 }
 
 // ---------------------------------------------------------------- host tables
@@ -236,6 +266,7 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
     const long nc = c->nc;
     hipStream_t s = c->stream;
     // P1 x-inverse
+    prof_begin(c, 0);
     st = with_size(c->Nx, [&](auto NXc) -> int {
         constexpr int NX = decltype(NXc)::value;
         constexpr int P = PXY<TB>;
@@ -250,8 +281,10 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
         }
         return NS3D_OK;
     });
+    prof_end(c);
     if (st) return st;
     // P2 y-inverse
+    prof_begin(c, 1);
     st = with_size(c->Ny, [&](auto NYc) -> int {
         constexpr int NY = decltype(NYc)::value;
         constexpr int P = PXY<TB>;
@@ -263,8 +296,10 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
             hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 2>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, c->tab, twp<TB>(c, 1));
         return NS3D_OK;
     });
+    prof_end(c);
     if (st) return st;
     // P3 z: C2R, product, R2C -> C at A
+    prof_begin(c, 2);
     st = with_size(c->Nz, [&](auto NZc) -> int {
         constexpr int NZ = decltype(NZc)::value;
         constexpr int P = PZ<NZ>;
@@ -276,8 +311,10 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
             hipLaunchKernelGGL((k_z_product<TB, NZ, P, 2>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, c->tab, twp<TB>(c, 2), scale);
         return NS3D_OK;
     });
+    prof_end(c);
     if (st) return st;
     // P4 y-forward C -> D
+    prof_begin(c, 3);
     st = with_size(c->Ny, [&](auto NYc) -> int {
         constexpr int NY = decltype(NYc)::value;
         constexpr int P = PXY<TB>;
@@ -286,13 +323,15 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
         hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1>), grid, blk, 0, s, A, A + 3 * nc, 3, nc, c->Ny, c->Mz, nlt, twp<TB>(c, 1));
         return NS3D_OK;
     });
+    prof_end(c);
     return st;
 }
 
 template <typename TB, int MODE>
 int run_x_forward_epilogue(ns3d_ctx* c, const EpiArgs& ea) {
     const cplx<TB>* D = (const cplx<TB>*)c->scratch + 3 * c->nc;
-    return with_size(c->Nx, [&](auto NXc) -> int {
+    prof_begin(c, 4);
+    int st = with_size(c->Nx, [&](auto NXc) -> int {
         constexpr int NX = decltype(NXc)::value;
         constexpr int P = PXY<TB>;
         const int nlt = (c->Mz + P - 1) / P;
@@ -301,6 +340,8 @@ int run_x_forward_epilogue(ns3d_ctx* c, const EpiArgs& ea) {
                            twp<TB>(c, 0), ea);
         return NS3D_OK;
     });
+    prof_end(c);
+    return st;
 }
 
 EpiArgs make_epi(const ns3d_ctx* c) {
@@ -568,6 +609,7 @@ int ns3d_destroy(ns3d_ctx* c) {
     hipFree(c->real_scratch);
     hipFree(c->kry);
     if (c->red_host) hipHostFree(c->red_host);
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     delete c;
     return NS3D_OK;
 }
@@ -969,6 +1011,32 @@ void ns3d_jv_exponential(void* ctx, void* vec_in, void* vec_out) {
         st = ns3d_rk3_ssp_uv(c, u[0], u[1], u[2], v[0], v[1], v[2], f[0], f[1], f[2]);
     if (!st) st = ns3d_pack_full(c, v[0], v[1], v[2], (double*)vec_out);
     set_err(c, st);
+}
+
+// ---- pass timing
+int ns3d_profile_enable(ns3d_ctx* c, int on) {
+    if (!c) return NS3D_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->prof_on = on != 0;
+    c->ev_used.clear();
+    return NS3D_OK;
+}
+
+int ns3d_profile_read(ns3d_ctx* c, double* ms, long long* calls) {
+    if (!c || !ms || !calls) return NS3D_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < NPASS; i++) {
+        ms[i] = 0.0;
+        calls[i] = 0;
+    }
+    for (auto& u : c->ev_used) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, c->ev_pool[u.second], c->ev_pool[u.second + 1]));
+        ms[u.first] += t;
+        calls[u.first] += 1;
+    }
+    c->ev_used.clear();
+    return NS3D_OK;
 }
 
 // ---- a11 / a12
