@@ -23,15 +23,22 @@ template <typename T, int N> struct PtrArr { T* p[N]; };
 // consecutive z indices l; lanes p-fastest so each row load is contiguous.
 template <typename TB, int NX, int P, int NF>
 __global__ void __launch_bounds__(P* Plan<NX>::TPP)
-k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int nlt,
-             SpecTables tab, const cplx<TB>* __restrict__ tw) {
+k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int nlt, int nb,
+             SpecTables tab, const cplx<TB>* __restrict__ twg) {
     constexpr int TPP = Plan<NX>::TPP;
     using F = PencilFFT<TB, NX, TPP, +1, typename Plan<NX>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NX * P];
+    __shared__ cplx<TB> tw[NX];
+    const int bid = xcd_tile();
+    if (bid >= nb) return;
     const int tid = threadIdx.x, p = tid % P, t = tid / P;
-    const int k = blockIdx.x / nlt, lt = blockIdx.x % nlt;
+    const int k = bid / nlt, lt = bid % nlt;
     const int l = lt * P + p;
     const bool lv = l < Mz;
+    // a column (k,l) whose every x mode is masked stays unwritten: the y pass skips it
+    const bool keep = lv && column_kept(tab, k, l);
+    if (!__syncthreads_or(keep)) return;
+    stage_twiddles<TB, NX>(tw, twg);
     const long S = (long)Ny * Mz;
     const long base = (long)k * Mz + l;
 #pragma unroll 1
@@ -47,7 +54,7 @@ k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, lon
                 double kx = 0.0;
                 if (F::in_active(t, q)) {
                     kx = tab.kx[j];
-                    if (lv && mask_keep(tab, j, k, l)) s = src[base + j * S];
+                    if (keep && mask_keep(tab, j, k, l)) s = src[base + j * S];
                 }
                 v[q * F::R0 + r] = cvt<TB>(s);
                 d[q * F::R0 + r] = mk<TB>((TB)(-s.y * kx), (TB)(s.x * kx));
@@ -58,7 +65,7 @@ k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, lon
         cplx<TB>* __restrict__ od = A + (long)(s_ * 6 + 3 + b) * nc;
         F::run(v, lds, tw, t, p);
         F::run(d, lds, tw, t, p);
-        if (lv) {
+        if (keep) {
 #pragma unroll
             for (int q = 0; q < F::BL; q++) {
                 if (!F::out_active(t, q)) continue;
@@ -77,15 +84,20 @@ k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, lon
 // y-inverse: B(w_b) , B(dx w_b) , B(i ky w_b).  Block: one x index j and P z indices.
 template <typename TB, int NY, int P, int NF>
 __global__ void __launch_bounds__(P* Plan<NY>::TPP)
-k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, int Ny, int Mz, int nlt,
-             SpecTables tab, const cplx<TB>* __restrict__ tw) {
+k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, int Ny, int Mz, int nlt, int nb,
+             SpecTables tab, const cplx<TB>* __restrict__ twg) {
     constexpr int TPP = Plan<NY>::TPP;
     using F = PencilFFT<TB, NY, TPP, +1, typename Plan<NY>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NY * P];
+    __shared__ cplx<TB> tw[NY];
+    const int bid = xcd_tile();
+    if (bid >= nb) return;
     const int tid = threadIdx.x, p = tid % P, t = tid / P;
-    const int j = blockIdx.x / nlt, lt = blockIdx.x % nlt;
+    const int j = bid / nlt, lt = bid % nlt;
+    if (lt * P > tab.lmax) return;  // z-planes beyond lmax are zero; the z pass skips them
     const int l = lt * P + p;
-    const bool lv = l < Mz;
+    const bool lv = l < Mz && l <= tab.lmax;
+    stage_twiddles<TB, NY>(tw, twg);
     const long S = Mz;
     const long base = (long)j * Ny * Mz + l;
 #pragma unroll 1
@@ -104,7 +116,7 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
                     TB ky = 0;
                     if (F::in_active(t, q)) {
                         ky = (TB)tab.ky[kk];
-                        if (lv) s = src[base + kk * S];
+                        if (lv && column_kept(tab, kk, l)) s = src[base + kk * S];
                     }
                     v[q * F::R0 + r] = s;
                     d[q * F::R0 + r] = mk<TB>(-s.y * ky, s.x * ky);
@@ -137,7 +149,7 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
                 for (int r = 0; r < F::R0; r++) {
                     const int kk = F::in_index(t, q, r);
                     cplx<TB> s = mk<TB>(0, 0);
-                    if (F::in_active(t, q) && lv) s = src[base + kk * S];
+                    if (F::in_active(t, q) && lv && column_kept(tab, kk, l)) s = src[base + kk * S];
                     v[q * F::R0 + r] = s;
                 }
             }
@@ -159,7 +171,7 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
 // z pass on one (x,y) pencil: paired C2R transforms (two real fields per
 // complex FFT), the advective product, and paired R2C transforms.
 // C2R convention: x,y passes already done; only Re of l=0 and l=Nz/2 is used.
-template <typename TB, int NZ, int P>
+template <typename TB, int NZ, int P, bool TRUNC = false>
 struct ZPencil {
     static constexpr int TPP = Plan<NZ>::TPP;
     static constexpr int PAD = (sizeof(TB) == 8) ? 1 : 2;
@@ -184,7 +196,7 @@ struct ZPencil {
                 TB kz = 0;
                 if (FI::in_active(t, q)) {
                     kz = (TB)tab.kz[sl];
-                    if (valid) {
+                    if (valid && (!TRUNC || sl <= tab.lmax)) {
                         a = s1[sl];
                         b = s2[sl];
                     }
@@ -250,13 +262,18 @@ struct ZPencil {
 
 template <typename TB, int NZ, int P, int NF>
 __global__ void __launch_bounds__(P* Plan<NZ>::TPP)
-k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, long npen, int Mz, SpecTables tab,
-            const cplx<TB>* __restrict__ tw, TB scale) {
-    using Z = ZPencil<TB, NZ, P>;
+k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, long npen, int Mz, int nb,
+            SpecTables tab, const cplx<TB>* __restrict__ twg, TB scale) {
+    // B is zero on z-planes l > lmax (never written by the y pass): TRUNC skips them
+    using Z = ZPencil<TB, NZ, P, true>;
     constexpr int TPP = Z::TPP, NPT = Z::NPT;
     __shared__ cplx<TB> lds[P * (NZ + Z::PAD)];
+    __shared__ cplx<TB> tw[NZ];
+    const int bid = xcd_tile();
+    if (bid >= nb) return;
+    stage_twiddles<TB, NZ>(tw, twg);
     const int tid = threadIdx.x, t = tid % TPP, p = tid / TPP;
-    const long pen = (long)blockIdx.x * P + p;
+    const long pen = (long)bid * P + p;
     const bool valid = pen < npen;
     const long off = valid ? pen * Mz : 0;
     auto fld = [&](int f) { return Bf + (long)f * nc + off; };
@@ -312,13 +329,17 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
 // place or in place (each block reads its pencils completely before writing).
 template <typename TB, int NA, int P, int DIR, int AXIS>
 __global__ void __launch_bounds__(P* Plan<NA>::TPP)
-k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, int nlt,
-           const cplx<TB>* __restrict__ tw) {
+k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, int nlt, int nb,
+           const cplx<TB>* __restrict__ twg) {
     constexpr int TPP = Plan<NA>::TPP;
     using F = PencilFFT<TB, NA, TPP, DIR, typename Plan<NA>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NA * P];
+    __shared__ cplx<TB> tw[NA];
+    const int bid = xcd_tile();
+    if (bid >= nb) return;
+    stage_twiddles<TB, NA>(tw, twg);
     const int tid = threadIdx.x, p = tid % P, t = tid / P;
-    const int o = blockIdx.x / nlt, lt = blockIdx.x % nlt;
+    const int o = bid / nlt, lt = bid % nlt;
     const int l = lt * P + p;
     const bool lv = l < Mz;
     const long S = AXIS == 0 ? (long)Ny * Mz : (long)Mz;
@@ -437,13 +458,17 @@ __device__ __forceinline__ void ipq_rule(cplx<double>& Qx, cplx<double>& Qy, cpl
 
 template <typename TB, int NX, int P, int MODE>
 __global__ void __launch_bounds__(P* Plan<NX>::TPP)
-k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int nlt, SpecTables tab,
-            const cplx<TB>* __restrict__ tw, EpiArgs ea) {
+k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int nlt, int nb, SpecTables tab,
+            const cplx<TB>* __restrict__ twg, EpiArgs ea) {
     constexpr int TPP = Plan<NX>::TPP;
     using F = PencilFFT<TB, NX, TPP, -1, typename Plan<NX>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NX * P];
+    __shared__ cplx<TB> tw[NX];
+    const int bid = xcd_tile();
+    if (bid >= nb) return;
+    stage_twiddles<TB, NX>(tw, twg);
     const int tid = threadIdx.x, p = tid % P, t = tid / P;
-    const int k = blockIdx.x / nlt, lt = blockIdx.x % nlt;
+    const int k = bid / nlt, lt = bid % nlt;
     const int l = lt * P + p;
     const bool lv = l < Mz;
     const long S = (long)Ny * Mz;

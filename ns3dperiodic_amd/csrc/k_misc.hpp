@@ -8,13 +8,17 @@ namespace ns3d {
 // Two neighbouring (x,y) pencils share one complex FFT: Y = X_p0 + i X_p1.
 template <typename TB, int NZ, int P>
 __global__ void __launch_bounds__(P* Plan<NZ>::TPP)
-k_z_c2r(const cplx<TB>* __restrict__ in, TB* __restrict__ out, long npairs, int Mz, SpecTables tab,
-        const cplx<TB>* __restrict__ tw, TB scale) {
+k_z_c2r(const cplx<TB>* __restrict__ in, TB* __restrict__ out, long npairs, int Mz, int nb, SpecTables tab,
+        const cplx<TB>* __restrict__ twg, TB scale) {
     using Z = ZPencil<TB, NZ, P>;
     constexpr int TPP = Z::TPP;
     __shared__ cplx<TB> lds[P * (NZ + Z::PAD)];
+    __shared__ cplx<TB> tw[NZ];
+    const int bid = xcd_tile();
+    if (bid >= nb) return;
+    stage_twiddles<TB, NZ>(tw, twg);
     const int tid = threadIdx.x, t = tid % TPP, p = tid / TPP;
-    const long pair = (long)blockIdx.x * P + p;
+    const long pair = (long)bid * P + p;
     const bool valid = pair < npairs;
     const long p0 = valid ? 2 * pair : 0, p1 = p0 + 1;
     TB x1[Z::NPT], x2[Z::NPT];
@@ -33,12 +37,17 @@ k_z_c2r(const cplx<TB>* __restrict__ in, TB* __restrict__ out, long npairs, int 
 
 template <typename TB, int NZ, int P>
 __global__ void __launch_bounds__(P* Plan<NZ>::TPP)
-k_z_r2c(const TB* __restrict__ in, cplx<TB>* __restrict__ out, long npairs, int Mz, const cplx<TB>* __restrict__ tw) {
+k_z_r2c(const TB* __restrict__ in, cplx<TB>* __restrict__ out, long npairs, int Mz, int nb,
+        const cplx<TB>* __restrict__ twg) {
     using Z = ZPencil<TB, NZ, P>;
     constexpr int TPP = Z::TPP;
     __shared__ cplx<TB> lds[P * (NZ + Z::PAD)];
+    __shared__ cplx<TB> tw[NZ];
+    const int bid = xcd_tile();
+    if (bid >= nb) return;
+    stage_twiddles<TB, NZ>(tw, twg);
     const int tid = threadIdx.x, t = tid % TPP, p = tid / TPP;
-    const long pair = (long)blockIdx.x * P + p;
+    const long pair = (long)bid * P + p;
     const bool valid = pair < npairs;
     const long p0 = valid ? 2 * pair : 0, p1 = p0 + 1;
     TB q1[Z::NPT], q2[Z::NPT];

@@ -197,6 +197,9 @@ int build_tables(ns3d_ctx* c) {
     while (std::sqrt(s) >= c23) s = std::nextafter(s, 0.0);
     while (std::sqrt(std::nextafter(s, 1.0)) < c23) s = std::nextafter(s, 1.0);
     t.mask_smax = s;
+    t.lmax = -1;
+    for (int l = 0; l < Mz; l++)
+        if (mzv[l] <= s) t.lmax = l;
     return NS3D_OK;
 }
 
@@ -254,6 +257,8 @@ int ensure_fp64(ns3d_ctx* c, zd** p, int fields) {
 }
 
 inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
+// grid for the XCD-remapped pencil kernels: block count padded to a multiple of 8
+inline dim3 xgrid(long nb) { return dim3((unsigned)(((nb + 7) / 8) * 8)); }
 
 // ---------------------------------------------------------------- fused pipeline
 // Runs P1..P4 for NF vector fields and leaves D (3 fields) at scratch + 3*nc.
@@ -271,13 +276,14 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
         constexpr int NX = decltype(NXc)::value;
         constexpr int P = PXY<TB>;
         const int nlt = (c->Mz + P - 1) / P;
-        dim3 grid(c->Ny * nlt), blk(P * Plan<NX>::TPP);
+        const int nb = c->Ny * nlt;
+        dim3 grid = xgrid(nb), blk(P * Plan<NX>::TPP);
         if (nf == 1) {
             PtrArr<const zd, 3> w{{in[0], in[1], in[2]}};
-            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, c->tab, twp<TB>(c, 0));
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 0));
         } else {
             PtrArr<const zd, 6> w{{in[0], in[1], in[2], in[3], in[4], in[5]}};
-            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, c->tab, twp<TB>(c, 0));
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 0));
         }
         return NS3D_OK;
     });
@@ -289,11 +295,12 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
         constexpr int NY = decltype(NYc)::value;
         constexpr int P = PXY<TB>;
         const int nlt = (c->Mz + P - 1) / P;
-        dim3 grid(c->Nx * nlt), blk(P * Plan<NY>::TPP);
+        const int nb = c->Nx * nlt;
+        dim3 grid = xgrid(nb), blk(P * Plan<NY>::TPP);
         if (nf == 1)
-            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 1>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, c->tab, twp<TB>(c, 1));
+            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 1>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 1));
         else
-            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 2>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, c->tab, twp<TB>(c, 1));
+            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 2>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 1));
         return NS3D_OK;
     });
     prof_end(c);
@@ -303,12 +310,13 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
     st = with_size(c->Nz, [&](auto NZc) -> int {
         constexpr int NZ = decltype(NZc)::value;
         constexpr int P = PZ<NZ>;
-        dim3 grid(nblk(c->npen, P)), blk(P * Plan<NZ>::TPP);
+        const int nb = (int)nblk(c->npen, P);
+        dim3 grid = xgrid(nb), blk(P * Plan<NZ>::TPP);
         const TB scale = (TB)(1.0 / (1.0 * c->Nx * c->Ny * c->Nz));
         if (nf == 1)
-            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 1>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, c->tab, twp<TB>(c, 2), scale);
+            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 1>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, nb, c->tab, twp<TB>(c, 2), scale);
         else
-            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 2>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, c->tab, twp<TB>(c, 2), scale);
+            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 2>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, nb, c->tab, twp<TB>(c, 2), scale);
         return NS3D_OK;
     });
     prof_end(c);
@@ -319,8 +327,9 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
         constexpr int NY = decltype(NYc)::value;
         constexpr int P = PXY<TB>;
         const int nlt = (c->Mz + P - 1) / P;
-        dim3 grid(c->Nx * nlt), blk(P * Plan<NY>::TPP);
-        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1>), grid, blk, 0, s, A, A + 3 * nc, 3, nc, c->Ny, c->Mz, nlt, twp<TB>(c, 1));
+        const int nb = c->Nx * nlt;
+        dim3 grid = xgrid(nb), blk(P * Plan<NY>::TPP);
+        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1>), grid, blk, 0, s, A, A + 3 * nc, 3, nc, c->Ny, c->Mz, nlt, nb, twp<TB>(c, 1));
         return NS3D_OK;
     });
     prof_end(c);
@@ -335,8 +344,9 @@ int run_x_forward_epilogue(ns3d_ctx* c, const EpiArgs& ea) {
         constexpr int NX = decltype(NXc)::value;
         constexpr int P = PXY<TB>;
         const int nlt = (c->Mz + P - 1) / P;
-        dim3 grid(c->Ny * nlt), blk(P * Plan<NX>::TPP);
-        hipLaunchKernelGGL((k_x_fwd_epi<TB, NX, P, MODE>), grid, blk, 0, c->stream, D, c->nc, c->Ny, c->Mz, nlt, c->tab,
+        const int nb = c->Ny * nlt;
+        dim3 grid = xgrid(nb), blk(P * Plan<NX>::TPP);
+        hipLaunchKernelGGL((k_x_fwd_epi<TB, NX, P, MODE>), grid, blk, 0, c->stream, D, c->nc, c->Ny, c->Mz, nlt, nb, c->tab,
                            twp<TB>(c, 0), ea);
         return NS3D_OK;
     });
@@ -412,8 +422,8 @@ int c2r_3d(ns3d_ctx* c, const cplx<TB>* in, TB* out, cplx<TB>* tmp, bool scale) 
         constexpr int NX = decltype(N)::value;
         constexpr int P = PXY<TB>;
         const int nlt = (c->Mz + P - 1) / P;
-        hipLaunchKernelGGL((k_axis_c2c<TB, NX, P, +1, 0>), dim3(c->Ny * nlt), dim3(P * Plan<NX>::TPP), 0, s, in, tmp, 1, nc,
-                           c->Ny, c->Mz, nlt, twp<TB>(c, 0));
+        hipLaunchKernelGGL((k_axis_c2c<TB, NX, P, +1, 0>), xgrid(c->Ny * nlt), dim3(P * Plan<NX>::TPP), 0, s, in, tmp, 1, nc,
+                           c->Ny, c->Mz, nlt, c->Ny * nlt, twp<TB>(c, 0));
         return NS3D_OK;
     });
     if (st) return st;
@@ -421,8 +431,8 @@ int c2r_3d(ns3d_ctx* c, const cplx<TB>* in, TB* out, cplx<TB>* tmp, bool scale) 
         constexpr int NY = decltype(N)::value;
         constexpr int P = PXY<TB>;
         const int nlt = (c->Mz + P - 1) / P;
-        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, +1, 1>), dim3(c->Nx * nlt), dim3(P * Plan<NY>::TPP), 0, s, tmp, tmp, 1,
-                           nc, c->Ny, c->Mz, nlt, twp<TB>(c, 1));
+        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, +1, 1>), xgrid(c->Nx * nlt), dim3(P * Plan<NY>::TPP), 0, s, tmp, tmp, 1,
+                           nc, c->Ny, c->Mz, nlt, c->Nx * nlt, twp<TB>(c, 1));
         return NS3D_OK;
     });
     if (st) return st;
@@ -431,8 +441,9 @@ int c2r_3d(ns3d_ctx* c, const cplx<TB>* in, TB* out, cplx<TB>* tmp, bool scale) 
         constexpr int P = PZ<NZ>;
         const long npairs = c->npen / 2;
         const TB sc = scale ? (TB)(1.0 / (1.0 * c->Nx * c->Ny * c->Nz)) : (TB)1;
-        hipLaunchKernelGGL((k_z_c2r<TB, NZ, P>), dim3(nblk(npairs, P)), dim3(P * Plan<NZ>::TPP), 0, s, tmp, out, npairs,
-                           c->Mz, c->tab, twp<TB>(c, 2), sc);
+        const int nb = (int)nblk(npairs, P);
+        hipLaunchKernelGGL((k_z_c2r<TB, NZ, P>), xgrid(nb), dim3(P * Plan<NZ>::TPP), 0, s, tmp, out, npairs,
+                           c->Mz, nb, c->tab, twp<TB>(c, 2), sc);
         return NS3D_OK;
     });
 }
@@ -445,8 +456,9 @@ int r2c_3d(ns3d_ctx* c, const TB* in, cplx<TB>* out) {
         constexpr int NZ = decltype(N)::value;
         constexpr int P = PZ<NZ>;
         const long npairs = c->npen / 2;
-        hipLaunchKernelGGL((k_z_r2c<TB, NZ, P>), dim3(nblk(npairs, P)), dim3(P * Plan<NZ>::TPP), 0, s, in, out, npairs,
-                           c->Mz, twp<TB>(c, 2));
+        const int nb = (int)nblk(npairs, P);
+        hipLaunchKernelGGL((k_z_r2c<TB, NZ, P>), xgrid(nb), dim3(P * Plan<NZ>::TPP), 0, s, in, out, npairs,
+                           c->Mz, nb, twp<TB>(c, 2));
         return NS3D_OK;
     });
     if (st) return st;
@@ -454,8 +466,8 @@ int r2c_3d(ns3d_ctx* c, const TB* in, cplx<TB>* out) {
         constexpr int NY = decltype(N)::value;
         constexpr int P = PXY<TB>;
         const int nlt = (c->Mz + P - 1) / P;
-        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1>), dim3(c->Nx * nlt), dim3(P * Plan<NY>::TPP), 0, s, out, out, 1,
-                           nc, c->Ny, c->Mz, nlt, twp<TB>(c, 1));
+        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1>), xgrid(c->Nx * nlt), dim3(P * Plan<NY>::TPP), 0, s, out, out, 1,
+                           nc, c->Ny, c->Mz, nlt, c->Nx * nlt, twp<TB>(c, 1));
         return NS3D_OK;
     });
     if (st) return st;
@@ -463,8 +475,8 @@ int r2c_3d(ns3d_ctx* c, const TB* in, cplx<TB>* out) {
         constexpr int NX = decltype(N)::value;
         constexpr int P = PXY<TB>;
         const int nlt = (c->Mz + P - 1) / P;
-        hipLaunchKernelGGL((k_axis_c2c<TB, NX, P, -1, 0>), dim3(c->Ny * nlt), dim3(P * Plan<NX>::TPP), 0, s, out, out, 1,
-                           nc, c->Ny, c->Mz, nlt, twp<TB>(c, 0));
+        hipLaunchKernelGGL((k_axis_c2c<TB, NX, P, -1, 0>), xgrid(c->Ny * nlt), dim3(P * Plan<NX>::TPP), 0, s, out, out, 1,
+                           nc, c->Ny, c->Mz, nlt, c->Ny * nlt, twp<TB>(c, 0));
         return NS3D_OK;
     });
 }

@@ -48,12 +48,33 @@ struct SpecTables {
     const double* mz;       // [Mz]
     double k1k2, k1k3, k2k3;  // products of the base wavenumbers 2*pi/L
     double mask_smax;       // largest s with sqrt(s) < 2/3 (host libm sqrt)
+    int lmax;               // largest l whose z-plane keeps any mode (mz[l] <= mask_smax)
 };
 
 // 2/3 mask of the reference (Q2): sqrt(mx+my+mz) < 2/3 and M(0)=0.
 __device__ __forceinline__ bool mask_keep(const SpecTables& t, int j, int k, int l) {
     const double s = (t.mx[j] + t.my[k]) + t.mz[l];
     return (s <= t.mask_smax) && ((j | k | l) != 0);
+}
+
+// A whole x-pencil (k,l) is masked out iff its j=0 member is (mx >= 0, rounding monotone).
+__device__ __forceinline__ bool column_kept(const SpecTables& t, int k, int l) {
+    return (t.my[k] + t.mz[l]) <= t.mask_smax;
+}
+
+// XCD-aware block order: the dispatcher hands workgroup b to XCD b mod 8, so
+// logical tile (b mod 8)*(G/8) + b/8 keeps neighbouring tiles (which share
+// the 128-B lines their rows straddle) on one XCD and its L2.  G = gridDim.x
+// is a multiple of 8; ids >= nb are padding.
+__device__ __forceinline__ int xcd_tile() {
+    const int b = blockIdx.x, g = gridDim.x;
+    return (b & 7) * (g >> 3) + (b >> 3);
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void stage_twiddles(cplx<T>* __restrict__ dst, const cplx<T>* __restrict__ src) {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
 }
 
 // Reference AM_ab and din_diffusion at (j,k,l).

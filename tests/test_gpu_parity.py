@@ -216,3 +216,21 @@ def test_shapiro_config1():
     err = g.Shapiro_test_case(0.1, ur)
     print("[shapiro] L-inf error after 100 steps:", err)
     assert err < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(512, 16, 16), (16, 512, 16), (16, 16, 512), (16, 192, 96)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_large_axes(shape):
+    """Largest and radix-3 plans on each axis: FFT, nonlinear term and one RK3 step."""
+    from tests.gpu_util import to_dev, to_host
+    print(f"\n[case] grid {shape}", flush=True)
+    g, o = _make(shape)
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(o.rshape)
+    assert rel_l2(to_host([g.FFTN(to_dev([x])[0])])[0], np.fft.rfftn(x)) < 1e-14
+    u = _field(shape, seed=22)
+    assert rel_l2(to_host(g.calculate_convolution_2p3(to_dev(u))), o.nonlinear(u)) < TOL
+    ud = to_dev(u)
+    g.RK3_SSP(ud, None)
+    zero = [np.zeros(o.cshape, np.complex128)] * 3
+    assert rel_l2(to_host(ud), o.rk3(u, zero)) < TOL
