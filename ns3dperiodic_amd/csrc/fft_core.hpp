@@ -152,6 +152,23 @@ template <> struct Plan<256> { using R = Seq<4, 8, 8>; static constexpr int TPP 
 template <> struct Plan<384> { using R = Seq<3, 2, 8, 8>; static constexpr int TPP = 48; };
 template <> struct Plan<512> { using R = Seq<8, 8, 8>; static constexpr int TPP = 64; };
 
+// z-pass plans: 4 points per thread (TPP = N/4), radix-4 last so that the
+// inverse plan and its mirror-image forward plan share the product layout with
+// few live registers per thread.
+template <int N> struct ZPlan;
+template <> struct ZPlan<8> { using R = Seq<2, 4>; static constexpr int TPP = 2; };
+template <> struct ZPlan<16> { using R = Seq<4, 4>; static constexpr int TPP = 4; };
+template <> struct ZPlan<24> { using R = Seq<3, 2, 4>; static constexpr int TPP = 6; };
+template <> struct ZPlan<32> { using R = Seq<2, 4, 4>; static constexpr int TPP = 8; };
+template <> struct ZPlan<48> { using R = Seq<3, 4, 4>; static constexpr int TPP = 12; };
+template <> struct ZPlan<64> { using R = Seq<4, 4, 4>; static constexpr int TPP = 16; };
+template <> struct ZPlan<96> { using R = Seq<3, 2, 4, 4>; static constexpr int TPP = 24; };
+template <> struct ZPlan<128> { using R = Seq<2, 4, 4, 4>; static constexpr int TPP = 32; };
+template <> struct ZPlan<192> { using R = Seq<3, 4, 4, 4>; static constexpr int TPP = 48; };
+template <> struct ZPlan<256> { using R = Seq<4, 4, 4, 4>; static constexpr int TPP = 64; };
+template <> struct ZPlan<384> { using R = Seq<3, 2, 4, 4, 4>; static constexpr int TPP = 96; };
+template <> struct ZPlan<512> { using R = Seq<2, 4, 4, 4, 4>; static constexpr int TPP = 128; };
+
 // Registers per thread for a plan: max over passes of ceil((N/R)/TPP)*R.
 template <int N, int TPP, typename S> struct PlanRegs;
 template <int N, int TPP, int... R> struct PlanRegs<N, TPP, Seq<R...>> {

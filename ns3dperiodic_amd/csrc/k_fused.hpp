@@ -18,15 +18,32 @@ namespace ns3d {
 
 template <typename T, int N> struct PtrArr { T* p[N]; };
 
+// Minimum waves per SIMD requested from the register allocator (the passes are
+// latency bound: more resident waves hide HBM latency better than ILP).
+#ifndef NS3D_WAVES_XY
+#define NS3D_WAVES_XY 2
+#endif
+#ifndef NS3D_WAVES_Z
+#define NS3D_WAVES_Z 2
+#endif
+// x/y-pass plan: E = 8 points per thread (Plan) or 4 (ZPlan); chosen per kernel
+// and precision by the host from measurements.
+template <int N, int E> struct XYSel { using type = Plan<N>; };
+template <int N> struct XYSel<N, 4> { using type = ZPlan<N>; };
+template <int N, int E> using XYPlan = typename XYSel<N, E>::type;
+// The inverse passes and the y-forward pass handle one field per block
+// (blockIdx.y): shorter blocks, more of them in flight.
+#define NS3D_FIELD_LOOP(f, n) for (int f = blockIdx.y; f < (int)blockIdx.y + 1; f++)
+
 // ---------------------------------------------------------------- P1
 // x-inverse of M w^_b and of M (i kx w^_b).  Block: one y index k and P
 // consecutive z indices l; lanes p-fastest so each row load is contiguous.
-template <typename TB, int NX, int P, int NF>
-__global__ void __launch_bounds__(P* Plan<NX>::TPP)
+template <typename TB, int NX, int P, int NF, int E>
+__global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
 k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int nlt, int nb,
              SpecTables tab, const cplx<TB>* __restrict__ twg) {
-    constexpr int TPP = Plan<NX>::TPP;
-    using F = PencilFFT<TB, NX, TPP, +1, typename Plan<NX>::R, IndexMajor<P>>;
+    constexpr int TPP = XYPlan<NX, E>::TPP;
+    using F = PencilFFT<TB, NX, TPP, +1, typename XYPlan<NX, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NX * P];
     __shared__ cplx<TB> tw[NX];
     const int bid = xcd_tile();
@@ -42,7 +59,7 @@ k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, lon
     const long S = (long)Ny * Mz;
     const long base = (long)k * Mz + l;
 #pragma unroll 1
-    for (int f = 0; f < 3 * NF; f++) {
+    NS3D_FIELD_LOOP(f, 3 * NF) {
         const cplx<double>* __restrict__ src = w.p[f];
         cplx<TB> v[F::E], d[F::E];
 #pragma unroll
@@ -82,12 +99,12 @@ k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, lon
 
 // ---------------------------------------------------------------- P2
 // y-inverse: B(w_b) , B(dx w_b) , B(i ky w_b).  Block: one x index j and P z indices.
-template <typename TB, int NY, int P, int NF>
-__global__ void __launch_bounds__(P* Plan<NY>::TPP)
+template <typename TB, int NY, int P, int NF, int E>
+__global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NY, E>::TPP))
 k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, int Ny, int Mz, int nlt, int nb,
              SpecTables tab, const cplx<TB>* __restrict__ twg) {
-    constexpr int TPP = Plan<NY>::TPP;
-    using F = PencilFFT<TB, NY, TPP, +1, typename Plan<NY>::R, IndexMajor<P>>;
+    constexpr int TPP = XYPlan<NY, E>::TPP;
+    using F = PencilFFT<TB, NY, TPP, +1, typename XYPlan<NY, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NY * P];
     __shared__ cplx<TB> tw[NY];
     const int bid = xcd_tile();
@@ -101,7 +118,7 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
     const long S = Mz;
     const long base = (long)j * Ny * Mz + l;
 #pragma unroll 1
-    for (int f = 0; f < 3 * NF; f++) {
+    NS3D_FIELD_LOOP(f, 3 * NF) {
         const int s_ = f / 3, b = f % 3;
         // velocity component: produce w_b and dy w_b
         {
@@ -173,11 +190,11 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
 // C2R convention: x,y passes already done; only Re of l=0 and l=Nz/2 is used.
 template <typename TB, int NZ, int P, bool TRUNC = false>
 struct ZPencil {
-    static constexpr int TPP = Plan<NZ>::TPP;
+    static constexpr int TPP = ZPlan<NZ>::TPP;
     static constexpr int PAD = (sizeof(TB) == 8) ? 1 : 2;
     using LA = PencilMajor<NZ, PAD>;
-    using FI = PencilFFT<TB, NZ, TPP, +1, typename Plan<NZ>::R, LA>;
-    using FF = PencilFFT<TB, NZ, TPP, -1, typename SeqRev<typename Plan<NZ>::R>::type, LA>;
+    using FI = PencilFFT<TB, NZ, TPP, +1, typename ZPlan<NZ>::R, LA>;
+    using FF = PencilFFT<TB, NZ, TPP, -1, typename SeqRev<typename ZPlan<NZ>::R>::type, LA>;
     static_assert(FI::RL == FF::R0 && FI::BL == FF::B0, "mirror plans must share the product layout");
     static constexpr int E = FI::E > FF::E ? FI::E : FF::E;
     static constexpr int NPT = FI::BL * FI::RL;  // physical points per thread
@@ -261,7 +278,7 @@ struct ZPencil {
 };
 
 template <typename TB, int NZ, int P, int NF>
-__global__ void __launch_bounds__(P* Plan<NZ>::TPP)
+__global__ void __attribute__((amdgpu_waves_per_eu(NF == 1 ? NS3D_WAVES_Z : 2, 8))) __launch_bounds__(P* ZPlan<NZ>::TPP)
 k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, long npen, int Mz, int nb,
             SpecTables tab, const cplx<TB>* __restrict__ twg, TB scale) {
     // B is zero on z-planes l > lmax (never written by the y pass): TRUNC skips them
@@ -327,12 +344,12 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
 // ---------------------------------------------------------------- P4 / generic
 // C2C transform along x (AXIS 0) or y (AXIS 1) of nf spectral fields, out of
 // place or in place (each block reads its pencils completely before writing).
-template <typename TB, int NA, int P, int DIR, int AXIS>
-__global__ void __launch_bounds__(P* Plan<NA>::TPP)
+template <typename TB, int NA, int P, int DIR, int AXIS, int E>
+__global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NA, E>::TPP))
 k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, int nlt, int nb,
            const cplx<TB>* __restrict__ twg) {
-    constexpr int TPP = Plan<NA>::TPP;
-    using F = PencilFFT<TB, NA, TPP, DIR, typename Plan<NA>::R, IndexMajor<P>>;
+    constexpr int TPP = XYPlan<NA, E>::TPP;
+    using F = PencilFFT<TB, NA, TPP, DIR, typename XYPlan<NA, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NA * P];
     __shared__ cplx<TB> tw[NA];
     const int bid = xcd_tile();
@@ -345,7 +362,7 @@ k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, i
     const long S = AXIS == 0 ? (long)Ny * Mz : (long)Mz;
     const long base = AXIS == 0 ? (long)o * Mz + l : (long)o * Ny * Mz + l;
 #pragma unroll 1
-    for (int f = 0; f < nf; f++) {
+    NS3D_FIELD_LOOP(f, nf) {
         const cplx<TB>* src = in + (long)f * nc;
         cplx<TB>* dst = out + (long)f * nc;
         cplx<TB> v[F::E];
@@ -456,12 +473,12 @@ __device__ __forceinline__ void ipq_rule(cplx<double>& Qx, cplx<double>& Qy, cpl
     }
 }
 
-template <typename TB, int NX, int P, int MODE>
-__global__ void __launch_bounds__(P* Plan<NX>::TPP)
+template <typename TB, int NX, int P, int MODE, int E>
+__global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
 k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int nlt, int nb, SpecTables tab,
             const cplx<TB>* __restrict__ twg, EpiArgs ea) {
-    constexpr int TPP = Plan<NX>::TPP;
-    using F = PencilFFT<TB, NX, TPP, -1, typename Plan<NX>::R, IndexMajor<P>>;
+    constexpr int TPP = XYPlan<NX, E>::TPP;
+    using F = PencilFFT<TB, NX, TPP, -1, typename XYPlan<NX, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NX * P];
     __shared__ cplx<TB> tw[NX];
     const int bid = xcd_tile();

@@ -42,9 +42,17 @@ bool size_ok(int n) {
 }
 
 // pencils per block in the x/y passes: rows of 128 B
-template <typename TB> constexpr int PXY = sizeof(TB) == 8 ? 8 : 16;
+template <typename TB> constexpr int PXY0 = sizeof(TB) == 8 ? 8 : 16;
+template <typename TB, int N, int E>
+constexpr int PXY = PXY0<TB> * XYPlan<N, E>::TPP <= 1024 ? PXY0<TB> : 1024 / XYPlan<N, E>::TPP;
+// Points per thread of each x/y pass, by precision (measured at 256^3 on MI355X):
+// FP64 favours 4 points in the inverse passes, 8 in the forward ones; FP32 the mix below.
+template <typename TB> constexpr int E_XINV = sizeof(TB) == 8 ? 4 : 8;
+template <typename TB> constexpr int E_YINV = 4;
+template <typename TB> constexpr int E_C2C = 8;
+template <typename TB> constexpr int E_XFWD = sizeof(TB) == 8 ? 8 : 4;
 // pencils per block in the z pass: ~128 threads
-template <int N> constexpr int PZ = (128 / Plan<N>::TPP) > 0 ? (128 / Plan<N>::TPP) : 1;
+template <int N> constexpr int PZ = (128 / ZPlan<N>::TPP) > 0 ? (128 / ZPlan<N>::TPP) : 1;
 
 #define HIPCHK(x)                                         \
     do {                                                  \
@@ -258,7 +266,7 @@ int ensure_fp64(ns3d_ctx* c, zd** p, int fields) {
 
 inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
 // grid for the XCD-remapped pencil kernels: block count padded to a multiple of 8
-inline dim3 xgrid(long nb) { return dim3((unsigned)(((nb + 7) / 8) * 8)); }
+inline dim3 xgrid(long nb, int nfy = 1) { return dim3((unsigned)(((nb + 7) / 8) * 8), (unsigned)nfy); }
 
 // ---------------------------------------------------------------- fused pipeline
 // Runs P1..P4 for NF vector fields and leaves D (3 fields) at scratch + 3*nc.
@@ -274,16 +282,17 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
     prof_begin(c, 0);
     st = with_size(c->Nx, [&](auto NXc) -> int {
         constexpr int NX = decltype(NXc)::value;
-        constexpr int P = PXY<TB>;
+        constexpr int E = E_XINV<TB>;
+        constexpr int P = PXY<TB, NX, E>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Ny * nlt;
-        dim3 grid = xgrid(nb), blk(P * Plan<NX>::TPP);
+        dim3 grid = xgrid(nb, 3 * nf), blk(P * XYPlan<NX, E>::TPP);
         if (nf == 1) {
             PtrArr<const zd, 3> w{{in[0], in[1], in[2]}};
-            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 0));
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 0));
         } else {
             PtrArr<const zd, 6> w{{in[0], in[1], in[2], in[3], in[4], in[5]}};
-            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 0));
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2, E>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 0));
         }
         return NS3D_OK;
     });
@@ -293,14 +302,15 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
     prof_begin(c, 1);
     st = with_size(c->Ny, [&](auto NYc) -> int {
         constexpr int NY = decltype(NYc)::value;
-        constexpr int P = PXY<TB>;
+        constexpr int E = E_YINV<TB>;
+        constexpr int P = PXY<TB, NY, E>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Nx * nlt;
-        dim3 grid = xgrid(nb), blk(P * Plan<NY>::TPP);
+        dim3 grid = xgrid(nb, 3 * nf), blk(P * XYPlan<NY, E>::TPP);
         if (nf == 1)
-            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 1>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 1));
+            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 1, E>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 1));
         else
-            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 2>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 1));
+            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 2, E>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 1));
         return NS3D_OK;
     });
     prof_end(c);
@@ -311,7 +321,7 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
         constexpr int NZ = decltype(NZc)::value;
         constexpr int P = PZ<NZ>;
         const int nb = (int)nblk(c->npen, P);
-        dim3 grid = xgrid(nb), blk(P * Plan<NZ>::TPP);
+        dim3 grid = xgrid(nb), blk(P * ZPlan<NZ>::TPP);
         const TB scale = (TB)(1.0 / (1.0 * c->Nx * c->Ny * c->Nz));
         if (nf == 1)
             hipLaunchKernelGGL((k_z_product<TB, NZ, P, 1>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, nb, c->tab, twp<TB>(c, 2), scale);
@@ -325,11 +335,12 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
     prof_begin(c, 3);
     st = with_size(c->Ny, [&](auto NYc) -> int {
         constexpr int NY = decltype(NYc)::value;
-        constexpr int P = PXY<TB>;
+        constexpr int E = E_C2C<TB>;
+        constexpr int P = PXY<TB, NY, E>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Nx * nlt;
-        dim3 grid = xgrid(nb), blk(P * Plan<NY>::TPP);
-        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1>), grid, blk, 0, s, A, A + 3 * nc, 3, nc, c->Ny, c->Mz, nlt, nb, twp<TB>(c, 1));
+        dim3 grid = xgrid(nb, 3), blk(P * XYPlan<NY, E>::TPP);
+        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1, E>), grid, blk, 0, s, A, A + 3 * nc, 3, nc, c->Ny, c->Mz, nlt, nb, twp<TB>(c, 1));
         return NS3D_OK;
     });
     prof_end(c);
@@ -342,11 +353,12 @@ int run_x_forward_epilogue(ns3d_ctx* c, const EpiArgs& ea) {
     prof_begin(c, 4);
     int st = with_size(c->Nx, [&](auto NXc) -> int {
         constexpr int NX = decltype(NXc)::value;
-        constexpr int P = PXY<TB>;
+        constexpr int E = E_XFWD<TB>;
+        constexpr int P = PXY<TB, NX, E>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Ny * nlt;
-        dim3 grid = xgrid(nb), blk(P * Plan<NX>::TPP);
-        hipLaunchKernelGGL((k_x_fwd_epi<TB, NX, P, MODE>), grid, blk, 0, c->stream, D, c->nc, c->Ny, c->Mz, nlt, nb, c->tab,
+        dim3 grid = xgrid(nb), blk(P * XYPlan<NX, E>::TPP);
+        hipLaunchKernelGGL((k_x_fwd_epi<TB, NX, P, MODE, E>), grid, blk, 0, c->stream, D, c->nc, c->Ny, c->Mz, nlt, nb, c->tab,
                            twp<TB>(c, 0), ea);
         return NS3D_OK;
     });
@@ -420,18 +432,20 @@ int c2r_3d(ns3d_ctx* c, const cplx<TB>* in, TB* out, cplx<TB>* tmp, bool scale) 
     const long nc = c->nc;
     int st = with_size(c->Nx, [&](auto N) -> int {
         constexpr int NX = decltype(N)::value;
-        constexpr int P = PXY<TB>;
+        constexpr int E = E_C2C<TB>;
+        constexpr int P = PXY<TB, NX, E>;
         const int nlt = (c->Mz + P - 1) / P;
-        hipLaunchKernelGGL((k_axis_c2c<TB, NX, P, +1, 0>), xgrid(c->Ny * nlt), dim3(P * Plan<NX>::TPP), 0, s, in, tmp, 1, nc,
+        hipLaunchKernelGGL((k_axis_c2c<TB, NX, P, +1, 0, E>), xgrid(c->Ny * nlt), dim3(P * XYPlan<NX, E>::TPP), 0, s, in, tmp, 1, nc,
                            c->Ny, c->Mz, nlt, c->Ny * nlt, twp<TB>(c, 0));
         return NS3D_OK;
     });
     if (st) return st;
     st = with_size(c->Ny, [&](auto N) -> int {
         constexpr int NY = decltype(N)::value;
-        constexpr int P = PXY<TB>;
+        constexpr int E = E_C2C<TB>;
+        constexpr int P = PXY<TB, NY, E>;
         const int nlt = (c->Mz + P - 1) / P;
-        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, +1, 1>), xgrid(c->Nx * nlt), dim3(P * Plan<NY>::TPP), 0, s, tmp, tmp, 1,
+        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, +1, 1, E>), xgrid(c->Nx * nlt), dim3(P * XYPlan<NY, E>::TPP), 0, s, tmp, tmp, 1,
                            nc, c->Ny, c->Mz, nlt, c->Nx * nlt, twp<TB>(c, 1));
         return NS3D_OK;
     });
@@ -442,7 +456,7 @@ int c2r_3d(ns3d_ctx* c, const cplx<TB>* in, TB* out, cplx<TB>* tmp, bool scale) 
         const long npairs = c->npen / 2;
         const TB sc = scale ? (TB)(1.0 / (1.0 * c->Nx * c->Ny * c->Nz)) : (TB)1;
         const int nb = (int)nblk(npairs, P);
-        hipLaunchKernelGGL((k_z_c2r<TB, NZ, P>), xgrid(nb), dim3(P * Plan<NZ>::TPP), 0, s, tmp, out, npairs,
+        hipLaunchKernelGGL((k_z_c2r<TB, NZ, P>), xgrid(nb), dim3(P * ZPlan<NZ>::TPP), 0, s, tmp, out, npairs,
                            c->Mz, nb, c->tab, twp<TB>(c, 2), sc);
         return NS3D_OK;
     });
@@ -457,25 +471,27 @@ int r2c_3d(ns3d_ctx* c, const TB* in, cplx<TB>* out) {
         constexpr int P = PZ<NZ>;
         const long npairs = c->npen / 2;
         const int nb = (int)nblk(npairs, P);
-        hipLaunchKernelGGL((k_z_r2c<TB, NZ, P>), xgrid(nb), dim3(P * Plan<NZ>::TPP), 0, s, in, out, npairs,
+        hipLaunchKernelGGL((k_z_r2c<TB, NZ, P>), xgrid(nb), dim3(P * ZPlan<NZ>::TPP), 0, s, in, out, npairs,
                            c->Mz, nb, twp<TB>(c, 2));
         return NS3D_OK;
     });
     if (st) return st;
     st = with_size(c->Ny, [&](auto N) -> int {
         constexpr int NY = decltype(N)::value;
-        constexpr int P = PXY<TB>;
+        constexpr int E = E_C2C<TB>;
+        constexpr int P = PXY<TB, NY, E>;
         const int nlt = (c->Mz + P - 1) / P;
-        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1>), xgrid(c->Nx * nlt), dim3(P * Plan<NY>::TPP), 0, s, out, out, 1,
+        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1, E>), xgrid(c->Nx * nlt), dim3(P * XYPlan<NY, E>::TPP), 0, s, out, out, 1,
                            nc, c->Ny, c->Mz, nlt, c->Nx * nlt, twp<TB>(c, 1));
         return NS3D_OK;
     });
     if (st) return st;
     return with_size(c->Nx, [&](auto N) -> int {
         constexpr int NX = decltype(N)::value;
-        constexpr int P = PXY<TB>;
+        constexpr int E = E_C2C<TB>;
+        constexpr int P = PXY<TB, NX, E>;
         const int nlt = (c->Mz + P - 1) / P;
-        hipLaunchKernelGGL((k_axis_c2c<TB, NX, P, -1, 0>), xgrid(c->Ny * nlt), dim3(P * Plan<NX>::TPP), 0, s, out, out, 1,
+        hipLaunchKernelGGL((k_axis_c2c<TB, NX, P, -1, 0, E>), xgrid(c->Ny * nlt), dim3(P * XYPlan<NX, E>::TPP), 0, s, out, out, 1,
                            nc, c->Ny, c->Mz, nlt, c->Ny * nlt, twp<TB>(c, 0));
         return NS3D_OK;
     });

@@ -1,0 +1,47 @@
+"""The C ABI library loads and exports every symbol include/ns3d.h declares (no GPU calls)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ns3d.h")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ns3d_\w+)\s*\(", src)))
+
+
+def test_header_and_binding_agree():
+    from ns3dperiodic_amd import _lib
+    assert set(declared()) == set(_lib.SIGNATURES), set(declared()) ^ set(_lib.SIGNATURES)
+
+
+def test_library_exports_every_symbol():
+    from ns3dperiodic_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libns3d_hip.so not built")
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (ns3d_\w+)", out))
+    missing = set(declared()) - exported
+    assert not missing, missing
+    lib = _lib.load()  # dlopen only; no HIP call is made
+    assert lib.ns3d_status_string(0) == b"ok"
+    assert lib.ns3d_supported_size(256) == 1 and lib.ns3d_supported_size(100) == 0
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    from ns3dperiodic_amd import _lib
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "absent.so"))
+    with pytest.raises(_lib.NS3DError):
+        _lib.load()
+
+
+def test_bench_pass_bytes_total_159_fields():
+    import bench
+    nc = 256 * 256 * 129
+    assert bench.step_bytes(nc, 16) == pytest.approx(159 * 16 * nc)

@@ -1,0 +1,228 @@
+"""Pin the oracle's nonlinear term, update family, RK3 integrator and J.v (CPU only).
+
+Checks come from what the mathematics fixes: analytic (u.grad)u of
+trigonometric fields, bilinearity identities, the Beltrami property, the
+SSP-RK3 stability/forcing polynomials on linear dynamics, the Shapiro exact
+solution, the laminar Kolmogorov fixed point of the reference's own initial
+condition, and central differences (exact for a quadratic map) for J.v.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from ns3dperiodic_amd import fields
+from tests.conftest import rel_l2
+
+
+def _grid(N):
+    return fields.grid(N, N, N)
+
+
+def test_nonlinear_analytic_with_and_without_ipq(oracle_mod):
+    """u = (sin y cos z, cos x, sin(x+z)): every product mode lies inside the mask."""
+    N = 16
+    X, Y, Z = _grid(N)
+    u = [np.sin(Y) * np.cos(Z), np.cos(X), np.sin(X + Z)]
+    d = {  # partial derivatives d_a u_b
+        (0, 0): 0 * X, (1, 0): np.cos(Y) * np.cos(Z), (2, 0): -np.sin(Y) * np.sin(Z),
+        (0, 1): -np.sin(X), (1, 1): 0 * X, (2, 1): 0 * X,
+        (0, 2): np.cos(X + Z), (1, 2): 0 * X, (2, 2): np.cos(X + Z)}
+    Q = [sum(u[a] * d[(a, b)] for a in range(3)) for b in range(3)]
+    o_raw = oracle_mod.Oracle(N, ipq=False)
+    uh = [o_raw.fft_r2c(c) for c in u]
+    Qh = [o_raw.fft_r2c(q) for q in Q]
+    assert rel_l2(o_raw.nonlinear(uh), Qh) < 1e-13
+    # IPQ rule (advection_2_3.cu:35-70): real parts and the mean mode zeroed
+    o = oracle_mod.Oracle(N, ipq=True)
+    expect = []
+    for q in Qh:
+        e = 1j * q.imag
+        e[0, 0, 0] = 0
+        expect.append(e)
+    assert any(np.abs(q.real).max() > 1 for q in Qh)  # the rule is exercised
+    assert rel_l2(o.nonlinear(uh), expect) < 1e-13
+
+
+def test_nonlinear_sign_and_component_order(oracle_mod):
+    """(u.grad)u for u = (sin y, 0, 0) is 0; for u = (0, sin x, 0) it is (0, 0, 0);
+    for u = (cos y, sin x, 0): Q_x = sin x * (-sin y), Q_y = cos y * cos x."""
+    N = 16
+    X, Y, Z = _grid(N)
+    o = oracle_mod.Oracle(N, ipq=False)
+    u = [np.cos(Y), np.sin(X), 0 * X]
+    Q = o.nonlinear([o.fft_r2c(c) for c in u])
+    q = [o.fft_c2r(c) for c in Q]
+    np.testing.assert_allclose(q[0], np.sin(X) * -np.sin(Y), atol=1e-13)
+    np.testing.assert_allclose(q[1], np.cos(Y) * np.cos(X), atol=1e-13)
+    np.testing.assert_allclose(q[2], 0, atol=1e-13)
+
+
+def test_bilinear_identities(oracle_mod):
+    N = 16
+    o = oracle_mod.Oracle(N)
+    u = fields.random_solenoidal(N, seed=1)
+    v = fields.random_solenoidal(N, seed=2, energy=0.2)
+    Nu, Nv = o.nonlinear(u), o.nonlinear(v)
+    Nuv = o.nonlinear([a + b for a, b in zip(u, v)])
+    B = o.bilinear(u, v)
+    assert rel_l2(B, [a - b - c for a, b, c in zip(Nuv, Nu, Nv)]) < 1e-12
+    assert rel_l2(o.bilinear(u, u), [2 * a for a in Nu]) < 1e-13
+
+
+def test_beltrami_nonlinear_term_is_projected_out(oracle_mod):
+    """Shapiro field: (u.grad)u is a gradient, so AM Q - Q = 0 (also after the IPQ rule)."""
+    N = 16
+    o = oracle_mod.Oracle(N, dt=1.0, Re=1e300)
+    u = [o.fft_r2c(c) for c in fields.shapiro(N)]
+    Q = o.nonlinear(u)
+    zero = [np.zeros(o.cshape, np.complex128)] * 3
+    # UV_RHS with v = 0: AM Q - Q
+    R = o.update(oracle_mod.UPD_UV_RHS, zero, Q)
+    assert max(np.abs(r).max() for r in R) < 1e-10 * max(np.abs(q).max() for q in Q)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+def test_update_family_linear_closed_forms(oracle_mod, mode):
+    """With Q = 0 each variant is a closed-form diagonal map of u (and f)."""
+    N = 8
+    dt, Re = 0.01, 7.0
+    o = oracle_mod.Oracle(N, dt=dt, Re=Re)
+    t = o.tables()
+    k2 = t["din_diffusion"]
+    rng = np.random.default_rng(3)
+    u = [rng.standard_normal(o.cshape) + 1j * rng.standard_normal(o.cshape) for _ in range(3)]
+    f = [rng.standard_normal(o.cshape) + 1j * rng.standard_normal(o.cshape) for _ in range(3)]
+    zero = [np.zeros(o.cshape, np.complex128)] * 3
+    out = o.update(mode, u, zero, f)
+    expect = {0: [(1 - k2 * dt / Re) * a + dt * b for a, b in zip(u, f)],
+              1: [-k2 / Re * a for a in u],
+              2: [(1 - k2 * dt / Re) * a for a in u],
+              3: [a / (1 + k2 * dt / Re) for a in u],
+              4: [-k2 / Re * a + b for a, b in zip(u, f)]}[mode]
+    assert rel_l2(out, expect) < 1e-14
+
+
+def test_update_projects_nonlinear_term(oracle_mod):
+    """The Q part of every variant is -dt*(I - AM)Q (stage forms) or -(I - AM)Q (RHS forms)."""
+    N = 8
+    dt = 0.02
+    o = oracle_mod.Oracle(N, dt=dt, Re=1e300)
+    t = o.tables()
+    rng = np.random.default_rng(4)
+    Q = [rng.standard_normal(o.cshape) + 1j * rng.standard_normal(o.cshape) for _ in range(3)]
+    zero = [np.zeros(o.cshape, np.complex128)] * 3
+    KX, KY, KZ = np.meshgrid(t["kx"], t["ky"], t["kz"], indexing="ij")
+    k2 = np.where(t["din_diffusion"] == 0, 1.0, t["din_diffusion"])
+    kq = (KX * Q[0] + KY * Q[1] + KZ * Q[2]) / k2
+    PQ = [Q[0] - KX * kq, Q[1] - KY * kq, Q[2] - KZ * kq]
+    for c in PQ:
+        c[0, 0, 0] = 0
+    PQ[0][0, 0, 0] = -Q[0][0, 0, 0] + Q[0][0, 0, 0]
+    # k = 0: AM = 0 there, so the update subtracts Q(0) itself
+    for c, q in zip(PQ, Q):
+        c[0, 0, 0] = q[0, 0, 0]
+    assert rel_l2(o.update(0, zero, Q, zero), [-dt * c for c in PQ]) < 1e-13
+    assert rel_l2(o.update(1, zero, Q), [-c for c in PQ]) < 1e-13
+
+
+def _shear(N, kk=2, amp=1.0):
+    """u = (amp sin(kk y), 0, 0): (u.grad)u = 0 identically."""
+    X, Y, Z = _grid(N)
+    return [amp * np.sin(kk * Y), 0 * X, 0 * X]
+
+
+def test_rk3_linear_decay_polynomial(oracle_mod):
+    """SSP-RK3 on a decoupled linear mode: factor 1 + z + z^2/2 + z^3/6 per step, z = -dt k^2/Re."""
+    # z = -0.2 on the mode, -2.4 at the corner mode (inside the RK3 stability interval)
+    N, dt, Re, kk = 16, 0.0125, 1.0, 4
+    o = oracle_mod.Oracle(N, dt=dt, Re=Re)
+    u = [o.fft_r2c(c) for c in _shear(N, kk)]
+    zero = [np.zeros(o.cshape, np.complex128)] * 3
+    out = o.rk3(u, zero, steps=5)
+    z = -dt * kk * kk / Re
+    g = 1 + z + z * z / 2 + z ** 3 / 6
+    assert rel_l2(out, [g ** 5 * c for c in u]) < 1e-14
+    # and not the plain explicit-Euler or exponential factors
+    assert abs(g - math.exp(z)) > 5e-5
+
+
+def test_rk3_constant_forcing_polynomial(oracle_mod):
+    """u' = -nu k^2 u + f: u1 = R(z) u0 + dt (1 + z/2 + z^2/6) f for SSP-RK3."""
+    N, dt, Re, kk = 16, 0.03, 3.0, 3
+    o = oracle_mod.Oracle(N, dt=dt, Re=Re)
+    u = [o.fft_r2c(c) for c in _shear(N, kk, 0.3)]
+    f = [o.fft_r2c(c) for c in _shear(N, kk, 1.7)]
+    out = o.rk3(u, f, steps=1)
+    z = -dt * kk * kk / Re
+    R = 1 + z + z * z / 2 + z ** 3 / 6
+    P = 1 + z / 2 + z * z / 6
+    assert rel_l2(out, [R * a + dt * P * b for a, b in zip(u, f)]) < 1e-14
+
+
+def test_shapiro_exact_solution(oracle_mod):
+    """Config 1 at reduced length: the Beltrami field decays as exp(-3 t/Re)."""
+    N, Re, dt = 16, 20.0, 1e-3
+    o = oracle_mod.Oracle(N, Re=Re, dt=dt)
+    u = [o.fft_r2c(c) for c in fields.shapiro(N)]
+    zero = [np.zeros(o.cshape, np.complex128)] * 3
+    out = o.rk3(u, zero, steps=20)
+    err = o.shapiro_error(20 * dt, [o.fft_c2r(c) for c in out])
+    assert err < 1e-13
+
+
+def test_reference_kolmogorov_fixed_point(oracle_mod):
+    """The reference's Fourier initial condition with its forcing is steady (n = 1)."""
+    N, Re = 16, 10.0
+    o = oracle_mod.Oracle(N, Re=Re, dt=0.01)
+    u, f = fields.reference_kolmogorov(N, n=1, Re=Re)
+    out = o.rk3(u, f, steps=10)
+    assert rel_l2(out, u) < 1e-13
+
+
+def test_jv_is_the_derivative_of_the_rhs(oracle_mod):
+    """J(u)v = d/de RHS(u + e v); central differences are exact for the quadratic map."""
+    N, Re = 8, 30.0
+    o = oracle_mod.Oracle(N, Re=Re, ipq=True)
+    u = fields.random_solenoidal(N, seed=5)
+    v = fields.random_solenoidal(N, seed=6, energy=1.0)
+    zero = [np.zeros(o.cshape, np.complex128)] * 3
+    eps = 1e-3
+    Rp = o.return_rhs([a + eps * b for a, b in zip(u, v)], zero)
+    Rm = o.return_rhs([a - eps * b for a, b in zip(u, v)], zero)
+    fd = [(a - b) / (2 * eps) for a, b in zip(Rp, Rm)]
+    assert rel_l2(o.rk3_uv_rhs(u, v), fd) < 1e-9
+
+
+def test_krylov_vector_round_trip(oracle_mod):
+    N = 8
+    o = oracle_mod.Oracle(N)
+    rng = np.random.default_rng(7)
+    vec = rng.standard_normal(3 * (N * N * (N // 2 + 1) - 1))
+    v = o.unpack_reduced(vec)
+    for c in v:
+        assert np.all(c.real == 0) and c[0, 0, 0] == 0
+    np.testing.assert_array_equal(o.pack_reduced(v), vec)
+
+
+def test_tangent_map_advances_base_and_is_linear(oracle_mod):
+    """RK3_SSP_UV: u moves by one RK3 step (Q10); the map is linear in v."""
+    N = 8
+    o = oracle_mod.Oracle(N, dt=5e-3, Re=50.0)
+    u = fields.random_solenoidal(N, seed=8)
+    v = fields.random_solenoidal(N, seed=9, energy=1e-2)
+    f = fields.kolmogorov_forcing_sin(N, n=1)
+    u1, v1 = o.rk3_uv(u, v, f)
+    assert rel_l2(u1, o.rk3(u, f)) < 1e-15
+    _, v2 = o.rk3_uv(u, [2.5 * c for c in v], f)
+    assert rel_l2(v2, [2.5 * c for c in v1]) < 1e-13
+
+
+def test_divergence_closed_form(oracle_mod):
+    N, dt = 8, 0.5
+    o = oracle_mod.Oracle(N, dt=dt)
+    X, Y, Z = _grid(N)
+    u = [np.sin(X), np.cos(2 * Y), np.sin(Z) * np.cos(X)]
+    div = o.divergence([o.fft_r2c(c) for c in u])
+    ref = (np.cos(X) - 2 * np.sin(2 * Y) + np.cos(Z) * np.cos(X)) / dt
+    np.testing.assert_allclose(o.fft_c2r(div), ref, atol=1e-12)
