@@ -45,3 +45,17 @@ def test_bench_pass_bytes_total_159_fields():
     import bench
     nc = 256 * 256 * 129
     assert bench.step_bytes(nc, 16) == pytest.approx(159 * 16 * nc)
+
+
+def test_headers_compile_as_c_and_cpp(tmp_path):
+    """ns3d.h is plain C99; ns3d_compat.hpp is host C++ with no HIP/CUDA dependency."""
+    c = tmp_path / "t.c"
+    c.write_text('#include "ns3d.h"\nint main(void){ ns3d_config cfg = {0}; (void)cfg; return 0; }\n')
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(c)],
+                   check=True)
+    cpp = tmp_path / "t.cpp"
+    cpp.write_text('#include "ns3d_compat.hpp"\n'
+                   'int main(){ ns3d_compat::user_map_vector f = ns3d_compat::NSCallMatrixVector_reduced();'
+                   ' (void)f; return ns3d_compat::bound() == nullptr ? 0 : 1; }\n')
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                    str(cpp)], check=True)
