@@ -130,6 +130,14 @@ int ns3d_weno_advect(ns3d_ctx *ctx, int scheme, const double *ux, const double *
 int ns3d_weno_advection(ns3d_ctx *ctx, int scheme, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
                         ns3d_cplx *Qx, ns3d_cplx *Qy, ns3d_cplx *Qz);             /* WenoAdvection */
 
+/* RK3_SSP of the velocity plus a passive scalar theta (physical, double [Nx][Ny][Nz]) advected
+ * with WENO (scheme 3/5) by each stage's 2/3-masked physical velocity, with the same SSP
+ * weights: theta_t = -(u.grad) theta.  No reference counterpart as a whole: it couples
+ * RK3_SSP (RK_time_step.cu:310-327) with the advection kernel of advection_WENO.cu:304-392
+ * applied to a scalar (the kernel's `x` argument). */
+int ns3d_rk3_ssp_scalar(ns3d_ctx *ctx, int nsteps, ns3d_cplx *ux, ns3d_cplx *uy, ns3d_cplx *uz,
+                        const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz, double *theta, int scheme);
+
 /* ---- a10: J.v and the tangent map (RK_time_step.cu:266-355, NS3D_periodic.cu:95-470) ---- */
 int ns3d_rk3_ssp_uv_rhs(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
                         ns3d_cplx *vx, ns3d_cplx *vy, ns3d_cplx *vz);             /* RK3_SSP_UV_RHS */

@@ -38,7 +38,9 @@ template <int N, int E> using XYPlan = typename XYSel<N, E>::type;
 // ---------------------------------------------------------------- P1
 // x-inverse of M w^_b and of M (i kx w^_b).  Block: one y index k and P
 // consecutive z indices l; lanes p-fastest so each row load is contiguous.
-template <typename TB, int NX, int P, int NF, int E>
+// grid.y enumerates the fields; with OS = 1 each block produces one output
+// (w_b or dx w_b) and the input is read by two blocks.
+template <typename TB, int NX, int P, int NF, int E, int OS>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
 k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int nlt, int nb,
              SpecTables tab, const cplx<TB>* __restrict__ twg) {
@@ -58,48 +60,54 @@ k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, lon
     stage_twiddles<TB, NX>(tw, twg);
     const long S = (long)Ny * Mz;
     const long base = (long)k * Mz + l;
-#pragma unroll 1
-    NS3D_FIELD_LOOP(f, 3 * NF) {
-        const cplx<double>* __restrict__ src = w.p[f];
-        cplx<TB> v[F::E], d[F::E];
+    const int f = OS ? (int)blockIdx.y >> 1 : (int)blockIdx.y;
+    const int which = OS ? (int)blockIdx.y & 1 : 0;  // OS: 0 -> w_b, 1 -> dx w_b
+    const cplx<double>* __restrict__ src = w.p[f];
+    cplx<TB> v[F::E], d[OS ? 1 : F::E];
 #pragma unroll
-        for (int q = 0; q < F::B0; q++) {
+    for (int q = 0; q < F::B0; q++) {
 #pragma unroll
-            for (int r = 0; r < F::R0; r++) {
-                const int j = F::in_index(t, q, r);
-                cplx<double> s = mk<double>(0.0, 0.0);
-                double kx = 0.0;
-                if (F::in_active(t, q)) {
-                    kx = tab.kx[j];
-                    if (keep && mask_keep(tab, j, k, l)) s = src[base + j * S];
-                }
+        for (int r = 0; r < F::R0; r++) {
+            const int j = F::in_index(t, q, r);
+            cplx<double> s = mk<double>(0.0, 0.0);
+            double kx = 0.0;
+            if (F::in_active(t, q)) {
+                kx = tab.kx[j];
+                if (keep && mask_keep(tab, j, k, l)) s = src[base + j * S];
+            }
+            const cplx<TB> dv = mk<TB>((TB)(-s.y * kx), (TB)(s.x * kx));
+            if constexpr (OS) {
+                v[q * F::R0 + r] = which ? dv : cvt<TB>(s);
+            } else {
                 v[q * F::R0 + r] = cvt<TB>(s);
-                d[q * F::R0 + r] = mk<TB>((TB)(-s.y * kx), (TB)(s.x * kx));
+                d[q * F::R0 + r] = dv;
             }
         }
-        const int s_ = f / 3, b = f % 3;
-        cplx<TB>* __restrict__ ow = A + (long)(s_ * 6 + b) * nc;
-        cplx<TB>* __restrict__ od = A + (long)(s_ * 6 + 3 + b) * nc;
-        F::run(v, lds, tw, t, p);
-        F::run(d, lds, tw, t, p);
-        if (keep) {
+    }
+    const int s_ = f / 3, b = f % 3;
+    cplx<TB>* __restrict__ ow = A + (long)(s_ * 6 + (OS ? 3 * which : 0) + b) * nc;
+    cplx<TB>* __restrict__ od = A + (long)(s_ * 6 + 3 + b) * nc;
+    F::run(v, lds, tw, t, p);
+    if constexpr (!OS) F::run(d, lds, tw, t, p);
+    if (keep) {
 #pragma unroll
-            for (int q = 0; q < F::BL; q++) {
-                if (!F::out_active(t, q)) continue;
+        for (int q = 0; q < F::BL; q++) {
+            if (!F::out_active(t, q)) continue;
 #pragma unroll
-                for (int r = 0; r < F::RL; r++) {
-                    const long g = base + F::out_index(t, q, r) * S;
-                    ow[g] = v[q * F::RL + r];
-                    od[g] = d[q * F::RL + r];
-                }
+            for (int r = 0; r < F::RL; r++) {
+                const long g = base + F::out_index(t, q, r) * S;
+                ow[g] = v[q * F::RL + r];
+                if constexpr (!OS) od[g] = d[q * F::RL + r];
             }
         }
     }
 }
 
 // ---------------------------------------------------------------- P2
-// y-inverse: B(w_b) , B(dx w_b) , B(i ky w_b).  Block: one x index j and P z indices.
-template <typename TB, int NY, int P, int NF, int E>
+// y-inverse: B(w_b) , B(dx w_b) , B(i ky w_b).  Block: one x index j and P z
+// indices.  grid.y enumerates components; with OS = 1 each block produces one
+// of the three outputs (A_b is then read by two blocks).
+template <typename TB, int NY, int P, int NF, int E, int OS>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NY, E>::TPP))
 k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, int Ny, int Mz, int nlt, int nb,
              SpecTables tab, const cplx<TB>* __restrict__ twg) {
@@ -117,68 +125,72 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
     stage_twiddles<TB, NY>(tw, twg);
     const long S = Mz;
     const long base = (long)j * Ny * Mz + l;
-#pragma unroll 1
-    NS3D_FIELD_LOOP(f, 3 * NF) {
-        const int s_ = f / 3, b = f % 3;
-        // velocity component: produce w_b and dy w_b
-        {
-            const cplx<TB>* __restrict__ src = A + (long)(s_ * 6 + b) * nc;
-            cplx<TB> v[F::E], d[F::E];
+    const int f = OS ? (int)blockIdx.y / 3 : (int)blockIdx.y;
+    const int which = OS ? (int)blockIdx.y % 3 : -1;  // OS: 0 -> w_b, 1 -> dy w_b, 2 -> dx w_b
+    const int s_ = f / 3, b = f % 3;
+    // velocity component: produce w_b and dy w_b
+    if (!OS || which < 2) {
+        const cplx<TB>* __restrict__ src = A + (long)(s_ * 6 + b) * nc;
+        cplx<TB> v[F::E], d[OS ? 1 : F::E];
 #pragma unroll
-            for (int q = 0; q < F::B0; q++) {
+        for (int q = 0; q < F::B0; q++) {
 #pragma unroll
-                for (int r = 0; r < F::R0; r++) {
-                    const int kk = F::in_index(t, q, r);
-                    cplx<TB> s = mk<TB>(0, 0);
-                    TB ky = 0;
-                    if (F::in_active(t, q)) {
-                        ky = (TB)tab.ky[kk];
-                        if (lv && column_kept(tab, kk, l)) s = src[base + kk * S];
-                    }
-                    v[q * F::R0 + r] = s;
-                    d[q * F::R0 + r] = mk<TB>(-s.y * ky, s.x * ky);
+            for (int r = 0; r < F::R0; r++) {
+                const int kk = F::in_index(t, q, r);
+                cplx<TB> s = mk<TB>(0, 0);
+                TB ky = 0;
+                if (F::in_active(t, q)) {
+                    ky = (TB)tab.ky[kk];
+                    if (lv && column_kept(tab, kk, l)) s = src[base + kk * S];
                 }
-            }
-            F::run(v, lds, tw, t, p);
-            F::run(d, lds, tw, t, p);
-            cplx<TB>* __restrict__ ow = B + (long)(s_ * 9 + b) * nc;
-            cplx<TB>* __restrict__ od = B + (long)(s_ * 9 + 6 + b) * nc;
-            if (lv) {
-#pragma unroll
-                for (int q = 0; q < F::BL; q++) {
-                    if (!F::out_active(t, q)) continue;
-#pragma unroll
-                    for (int r = 0; r < F::RL; r++) {
-                        const long g = base + F::out_index(t, q, r) * S;
-                        ow[g] = v[q * F::RL + r];
-                        od[g] = d[q * F::RL + r];
-                    }
+                const cplx<TB> dv = mk<TB>(-s.y * ky, s.x * ky);
+                if constexpr (OS) {
+                    v[q * F::R0 + r] = which ? dv : s;
+                } else {
+                    v[q * F::R0 + r] = s;
+                    d[q * F::R0 + r] = dv;
                 }
             }
         }
-        // x-derivative component: plain y-inverse
-        {
-            const cplx<TB>* __restrict__ src = A + (long)(s_ * 6 + 3 + b) * nc;
-            cplx<TB> v[F::E];
+        F::run(v, lds, tw, t, p);
+        if constexpr (!OS) F::run(d, lds, tw, t, p);
+        cplx<TB>* __restrict__ ow = B + (long)(s_ * 9 + (OS && which == 1 ? 6 : 0) + b) * nc;
+        cplx<TB>* __restrict__ od = B + (long)(s_ * 9 + 6 + b) * nc;
+        if (lv) {
 #pragma unroll
-            for (int q = 0; q < F::B0; q++) {
+            for (int q = 0; q < F::BL; q++) {
+                if (!F::out_active(t, q)) continue;
 #pragma unroll
-                for (int r = 0; r < F::R0; r++) {
-                    const int kk = F::in_index(t, q, r);
-                    cplx<TB> s = mk<TB>(0, 0);
-                    if (F::in_active(t, q) && lv && column_kept(tab, kk, l)) s = src[base + kk * S];
-                    v[q * F::R0 + r] = s;
+                for (int r = 0; r < F::RL; r++) {
+                    const long g = base + F::out_index(t, q, r) * S;
+                    ow[g] = v[q * F::RL + r];
+                    if constexpr (!OS) od[g] = d[q * F::RL + r];
                 }
             }
-            F::run(v, lds, tw, t, p);
-            cplx<TB>* __restrict__ ow = B + (long)(s_ * 9 + 3 + b) * nc;
-            if (lv) {
+        }
+    }
+    // x-derivative component: plain y-inverse
+    if (!OS || which == 2) {
+        const cplx<TB>* __restrict__ src = A + (long)(s_ * 6 + 3 + b) * nc;
+        cplx<TB> v[F::E];
 #pragma unroll
-                for (int q = 0; q < F::BL; q++) {
-                    if (!F::out_active(t, q)) continue;
+        for (int q = 0; q < F::B0; q++) {
 #pragma unroll
-                    for (int r = 0; r < F::RL; r++) ow[base + F::out_index(t, q, r) * S] = v[q * F::RL + r];
-                }
+            for (int r = 0; r < F::R0; r++) {
+                const int kk = F::in_index(t, q, r);
+                cplx<TB> s = mk<TB>(0, 0);
+                if (F::in_active(t, q) && lv && column_kept(tab, kk, l)) s = src[base + kk * S];
+                v[q * F::R0 + r] = s;
+            }
+        }
+        F::run(v, lds, tw, t, p);
+        cplx<TB>* __restrict__ ow = B + (long)(s_ * 9 + 3 + b) * nc;
+        if (lv) {
+#pragma unroll
+            for (int q = 0; q < F::BL; q++) {
+                if (!F::out_active(t, q)) continue;
+#pragma unroll
+                for (int r = 0; r < F::RL; r++) ow[base + F::out_index(t, q, r) * S] = v[q * F::RL + r];
             }
         }
     }
@@ -280,7 +292,7 @@ struct ZPencil {
 template <typename TB, int NZ, int P, int NF>
 __global__ void __attribute__((amdgpu_waves_per_eu(NF == 1 ? NS3D_WAVES_Z : 2, 8))) __launch_bounds__(P* ZPlan<NZ>::TPP)
 k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, long npen, int Mz, int nb,
-            SpecTables tab, const cplx<TB>* __restrict__ twg, TB scale) {
+            SpecTables tab, const cplx<TB>* __restrict__ twg, TB scale, TB* __restrict__ uphys, long nr) {
     // B is zero on z-planes l > lmax (never written by the y pass): TRUNC skips them
     using Z = ZPencil<TB, NZ, P, true>;
     constexpr int TPP = Z::TPP, NPT = Z::NPT;
@@ -300,6 +312,16 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
         TB w[3][NPT], a[NPT], b[NPT], keep[NPT];
         Z::c2r_pair(w[0], w[1], fld(0), false, fld(1), false, valid, tab, lds, tw, t, p, scale);
         Z::c2r_pair(w[2], keep, fld(2), false, fld(3), false, valid, tab, lds, tw, t, p, scale);  // w_z, dx w_x
+        if (uphys && valid) {
+            // the masked physical velocity, for a scalar advected by this stage's flow
+#pragma unroll
+            for (int e = 0; e < NPT; e++) {
+                const long g = pen * NZ + Z::FI::out_index(t, e / Z::FI::RL, e % Z::FI::RL);
+                uphys[g] = w[0][e];
+                uphys[nr + g] = w[1][e];
+                uphys[2 * nr + g] = w[2][e];
+            }
+        }
         Z::c2r_pair(a, b, fld(6), false, fld(0), true, valid, tab, lds, tw, t, p, scale);         // dy w_x, dz w_x
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[0][e] = w[0][e] * keep[e] + w[1][e] * a[e] + w[2][e] * b[e];

@@ -245,6 +245,38 @@ k_weno(const double* __restrict__ ux, const double* __restrict__ uy, const doubl
     out[id] = ux[id] * (se - sw) / dx + uy[id] * (ss - sn) / dy + uz[id] * (sf - sr) / dz;
 }
 
+// SSP stage of a passive scalar: out = a0*th_n + a1*(th_s - dt * (u.grad)_WENO th_s),
+// u being the masked physical velocity of the same stage (TU = bulk precision).
+template <typename TU>
+__global__ void __launch_bounds__(256)
+k_weno_stage(const TU* __restrict__ u, long nr, const double* __restrict__ x, const double* __restrict__ xn,
+             double* __restrict__ out, Grid3 G, double dx, double dy, double dz, int scheme, int zfix, double dt,
+             double a0, double a1) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nr) return;
+    const TU* ux = u;
+    const TU* uy = u + nr;
+    const TU* uz = u + 2 * nr;
+    const int l = (int)(id % G.Nz);
+    const long t1 = id / G.Nz;
+    const int k = (int)(t1 % G.Ny), j = (int)(t1 / G.Ny);
+    const double Uw = 0.5 * ((double)ux[G.at(j - 1, k, l)] + (double)ux[id]);
+    const double Ue = 0.5 * ((double)ux[id] + (double)ux[G.at(j + 1, k, l)]);
+    const double sw = Uw >= 0.0 ? weno_face<0>(x, G, j - 1, k, l, false, scheme, zfix) : weno_face<0>(x, G, j, k, l, true, scheme, zfix);
+    const double se = Ue >= 0.0 ? weno_face<0>(x, G, j, k, l, false, scheme, zfix) : weno_face<0>(x, G, j + 1, k, l, true, scheme, zfix);
+    const double Un = 0.5 * ((double)uy[G.at(j, k - 1, l)] + (double)uy[id]);
+    const double Us = 0.5 * ((double)uy[id] + (double)uy[G.at(j, k + 1, l)]);
+    const double sn = Un >= 0.0 ? weno_face<1>(x, G, j, k - 1, l, false, scheme, zfix) : weno_face<1>(x, G, j, k, l, true, scheme, zfix);
+    const double ss = Us >= 0.0 ? weno_face<1>(x, G, j, k, l, false, scheme, zfix) : weno_face<1>(x, G, j, k + 1, l, true, scheme, zfix);
+    const double Ur = 0.5 * ((double)uz[G.at(j, k, l - 1)] + (double)uz[id]);
+    const double Uf = 0.5 * ((double)uz[id] + (double)uz[G.at(j, k, l + 1)]);
+    const double sr = Ur >= 0.0 ? weno_face<2>(x, G, j, k, l - 1, false, scheme, zfix) : weno_face<2>(x, G, j, k, l, true, scheme, zfix);
+    const double sf = Uf >= 0.0 ? weno_face<2>(x, G, j, k, l, false, scheme, zfix) : weno_face<2>(x, G, j, k, l + 1, true, scheme, zfix);
+    const double adv = (double)ux[id] * (se - sw) / dx + (double)uy[id] * (ss - sn) / dy + (double)uz[id] * (sf - sr) / dz;
+    const double e = x[id] - dt * adv;
+    out[id] = xn ? a0 * xn[id] + a1 * e : a1 * e;
+}
+
 // ------------------------------------------------------- a11: reductions
 // Wave64 butterfly reductions via cross-lane shuffles; block partials, then
 // a single-block finish.  Sums are deterministic (fixed partition and order).

@@ -51,6 +51,13 @@ template <typename TB> constexpr int E_XINV = sizeof(TB) == 8 ? 4 : 8;
 template <typename TB> constexpr int E_YINV = 4;
 template <typename TB> constexpr int E_C2C = 8;
 template <typename TB> constexpr int E_XFWD = sizeof(TB) == 8 ? 8 : 4;
+// one output field per block in the inverse passes (input re-read by 2 or 3 blocks)
+#ifndef NS3D_OS_XINV
+#define NS3D_OS_XINV 0
+#endif
+#ifndef NS3D_OS_YINV
+#define NS3D_OS_YINV 0
+#endif
 // pencils per block in the z pass: ~128 threads
 template <int N> constexpr int PZ = (128 / ZPlan<N>::TPP) > 0 ? (128 / ZPlan<N>::TPP) : 1;
 
@@ -91,6 +98,10 @@ struct ns3d_ctx {
     int timesteps_period = 1;
     double tau = 1.0;
     zd* kry = nullptr;  // 3 fields for J.v unpacking
+    // passive scalar: physical velocity written by the z pass (bulk precision, 3 real fields)
+    void* uphys = nullptr;
+    void* uphys_out = nullptr;  // non-null only while a scalar step wants it
+    double* theta_stage = nullptr;  // 2 real FP64 fields
     // optional per-pass timing with HIP events on the context stream
     bool prof_on = false;
     std::vector<hipEvent_t> ev_pool;
@@ -286,13 +297,14 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
         constexpr int P = PXY<TB, NX, E>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Ny * nlt;
-        dim3 grid = xgrid(nb, 3 * nf), blk(P * XYPlan<NX, E>::TPP);
+        constexpr int OS = NS3D_OS_XINV;
+        dim3 grid = xgrid(nb, 3 * nf * (OS ? 2 : 1)), blk(P * XYPlan<NX, E>::TPP);
         if (nf == 1) {
             PtrArr<const zd, 3> w{{in[0], in[1], in[2]}};
-            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 0));
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E, OS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 0));
         } else {
             PtrArr<const zd, 6> w{{in[0], in[1], in[2], in[3], in[4], in[5]}};
-            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2, E>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 0));
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2, E, OS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 0));
         }
         return NS3D_OK;
     });
@@ -306,11 +318,12 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
         constexpr int P = PXY<TB, NY, E>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Nx * nlt;
-        dim3 grid = xgrid(nb, 3 * nf), blk(P * XYPlan<NY, E>::TPP);
+        constexpr int OS = NS3D_OS_YINV;
+        dim3 grid = xgrid(nb, 3 * nf * (OS ? 3 : 1)), blk(P * XYPlan<NY, E>::TPP);
         if (nf == 1)
-            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 1, E>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 1));
+            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 1, E, OS>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 1));
         else
-            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 2, E>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 1));
+            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 2, E, OS>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 1));
         return NS3D_OK;
     });
     prof_end(c);
@@ -323,10 +336,13 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
         const int nb = (int)nblk(c->npen, P);
         dim3 grid = xgrid(nb), blk(P * ZPlan<NZ>::TPP);
         const TB scale = (TB)(1.0 / (1.0 * c->Nx * c->Ny * c->Nz));
+        TB* up = nf == 1 ? (TB*)c->uphys_out : nullptr;
         if (nf == 1)
-            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 1>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, nb, c->tab, twp<TB>(c, 2), scale);
+            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 1>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, nb, c->tab, twp<TB>(c, 2), scale,
+                               up, c->nr);
         else
-            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 2>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, nb, c->tab, twp<TB>(c, 2), scale);
+            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 2>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, nb, c->tab, twp<TB>(c, 2), scale,
+                               (TB*)nullptr, c->nr);
         return NS3D_OK;
     });
     prof_end(c);
@@ -564,6 +580,44 @@ RedArgs red_args(const ns3d_ctx* c, const double* a, const double* b, const doub
     return r;
 }
 
+// ---------------------------------------------------------------- passive scalar RK3
+int weno_stage(ns3d_ctx* c, int scheme, const double* xs, const double* xn, double* out, double a0, double a1) {
+    Grid3 G{c->Nx, c->Ny, c->Nz};
+    const int zfix = (c->flags & NS3D_FLAG_WENO_ZFIX) ? 1 : 0;
+    const double dx = c->Lx / c->Nx, dy = c->Ly / c->Ny, dz = c->Lz / c->Nz;
+    if (c->precision == NS3D_FP64)
+        hipLaunchKernelGGL(k_weno_stage<double>, dim3(nblk(c->nr, 256)), dim3(256), 0, c->stream, (const double*)c->uphys,
+                           c->nr, xs, xn, out, G, dx, dy, dz, scheme, zfix, c->dt, a0, a1);
+    else
+        hipLaunchKernelGGL(k_weno_stage<float>, dim3(nblk(c->nr, 256)), dim3(256), 0, c->stream, (const float*)c->uphys,
+                           c->nr, xs, xn, out, G, dx, dy, dz, scheme, zfix, c->dt, a0, a1);
+    return hipGetLastError() == hipSuccess ? NS3D_OK : NS3D_ERR_HIP;
+}
+
+int rk3_scalar_one(ns3d_ctx* c, zd* const u[3], const zd* const f[3], double* th, int scheme) {
+    int st = ensure_fp64(c, &c->stage, 9);
+    if (st) return st;
+    if (!c->uphys && hipMalloc(&c->uphys, bulk_bytes(c) / 2 * (size_t)c->nr * 3) != hipSuccess) return NS3D_ERR_ALLOC;
+    if (!c->theta_stage && hipMalloc((void**)&c->theta_stage, sizeof(double) * (size_t)c->nr * 2) != hipSuccess)
+        return NS3D_ERR_ALLOC;
+    zd* u1[3] = {c->stage, c->stage + c->nc, c->stage + 2 * c->nc};
+    zd* u2[3] = {c->stage + 3 * c->nc, c->stage + 4 * c->nc, c->stage + 5 * c->nc};
+    const zd* uc[3] = {u[0], u[1], u[2]};
+    const zd* u1c[3] = {u1[0], u1[1], u1[2]};
+    const zd* u2c[3] = {u2[0], u2[1], u2[2]};
+    double* th1 = c->theta_stage;
+    double* th2 = c->theta_stage + c->nr;
+    c->uphys_out = c->uphys;
+    // the velocity stages write their physical flow; the scalar stage follows with the same SSP weights
+    if (!(st = stage_into(c, uc, nullptr, 0.0, 1.0, u1, f))) st = weno_stage(c, scheme, th, nullptr, th1, 0.0, 1.0);
+    if (!st && !(st = stage_into(c, u1c, uc, 3.0 / 4.0, 1.0 / 4.0, u2, f)))
+        st = weno_stage(c, scheme, th1, th, th2, 3.0 / 4.0, 1.0 / 4.0);
+    if (!st && !(st = stage_into(c, u2c, uc, 1.0 / 3.0, 2.0 / 3.0, u, f)))
+        st = weno_stage(c, scheme, th2, th, th, 1.0 / 3.0, 2.0 / 3.0);
+    c->uphys_out = nullptr;
+    return st;
+}
+
 }  // namespace
 
 // =====================================================================================
@@ -636,6 +690,8 @@ int ns3d_destroy(ns3d_ctx* c) {
     hipFree(c->red);
     hipFree(c->real_scratch);
     hipFree(c->kry);
+    hipFree(c->uphys);
+    hipFree(c->theta_stage);
     if (c->red_host) hipHostFree(c->red_host);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     delete c;
@@ -849,6 +905,18 @@ int ns3d_rk3_ssp_steps(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns
     const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
     for (int i = 0; i < nsteps; i++) {
         int st = rk3_one(c, u, f);
+        if (st) return st;
+    }
+    return NS3D_OK;
+}
+
+int ns3d_rk3_ssp_scalar(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns3d_cplx* uz, const ns3d_cplx* fx,
+                        const ns3d_cplx* fy, const ns3d_cplx* fz, double* theta, int scheme) {
+    if (!c || !ux || !uy || !uz || !theta || nsteps < 0) return NS3D_ERR_ARG;
+    zd* u[3] = {(zd*)ux, (zd*)uy, (zd*)uz};
+    const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
+    for (int i = 0; i < nsteps; i++) {
+        int st = rk3_scalar_one(c, u, f, theta, scheme);
         if (st) return st;
     }
     return NS3D_OK;

@@ -172,6 +172,13 @@ class NS3D:
         check(self.lib.ns3d_rk3_ssp_steps(self.ctx, steps, *self._c3(u), *self._f3(f)), "ns3d_rk3_ssp_steps")
         return u
 
+    def RK3_SSP_scalar(self, u, theta, f=None, steps=1, scheme=5):
+        """RK3 of the velocity plus a WENO-advected passive scalar theta (physical float64)."""
+        assert theta.dtype == torch.float64 and tuple(theta.shape) == self.rshape
+        check(self.lib.ns3d_rk3_ssp_scalar(self.ctx, steps, *self._c3(u), *self._f3(f), _ptr(theta), scheme),
+              "ns3d_rk3_ssp_scalar")
+        return u, theta
+
     # ------------------------------------------------------------ a8
     def divergence(self, u):
         d = self.cfield(1)

@@ -700,6 +700,54 @@ void or_weno_advection_spectral(const or_params *p, int scheme,
     free(u);
 }
 
+/* RK3_SSP (Source/RK_time_step.cu:310-327) with a passive scalar advected by the
+ * WENO kernel (advection_WENO.cu:304-392, scalar `x` argument) using each stage's
+ * 2/3-masked physical velocity C2R(M w^_s) and the same SSP weights. */
+static void scalar_stage(const or_params *p, int scheme, const double *w[3], const double *th_s,
+                         const double *th_n, double a0, double a1, double *out)
+{
+    tables_t t;
+    tables_make(p, &t);
+    const long nr = t.nr;
+    cx *sp = (cx *)malloc(sizeof(cx) * (size_t)t.nc);
+    double *up = (double *)malloc(sizeof(double) * (size_t)nr * 4);
+    for (int b = 0; b < 3; b++) {
+        masked_field_and_derivs(&t, (const cx *)w[b], sp, NULL, NULL, NULL);
+        or_fft_c2r(p, (const double *)sp, up + b * nr, 1);
+    }
+    or_weno_advect(p, scheme, up, up + nr, up + 2 * nr, th_s, up + 3 * nr);
+    for (long i = 0; i < nr; i++) {
+        const double e = th_s[i] - p->dt * up[3 * nr + i];
+        out[i] = th_n ? a0 * th_n[i] + a1 * e : a1 * e;
+    }
+    free(sp); free(up);
+    tables_free(&t);
+}
+
+void or_rk3_ssp_scalar(const or_params *p, double *ux, double *uy, double *uz,
+                       const double *fx, const double *fy, const double *fz, double *theta, int scheme)
+{
+    const long nc = ncomplex(p), nr = (long)p->Nx * p->Ny * p->Nz;
+    double *th1 = (double *)malloc(sizeof(double) * (size_t)nr * 2), *th2 = th1 + nr;
+    double *u1 = (double *)malloc(sizeof(cx) * (size_t)nc * 3);
+    double *u2 = (double *)malloc(sizeof(cx) * (size_t)nc * 3);
+    double *u[3] = {ux, uy, uz};
+    const double *w0[3] = {ux, uy, uz};
+    scalar_stage(p, scheme, w0, theta, NULL, 0.0, 1.0, th1);
+    for (int c = 0; c < 3; c++) memcpy(u1 + 2 * c * nc, u[c], sizeof(cx) * (size_t)nc);
+    or_single_time_step(p, u1, u1 + 2 * nc, u1 + 4 * nc, fx, fy, fz);
+    const double *w1[3] = {u1, u1 + 2 * nc, u1 + 4 * nc};
+    scalar_stage(p, scheme, w1, th1, theta, 3.0 / 4.0, 1.0 / 4.0, th2);
+    or_single_time_step(p, u1, u1 + 2 * nc, u1 + 4 * nc, fx, fy, fz);
+    for (int c = 0; c < 3; c++) axpby(nc, 3.0 / 4.0, u[c], 1.0 / 4.0, u1 + 2 * c * nc, u2 + 2 * c * nc);
+    const double *w2[3] = {u2, u2 + 2 * nc, u2 + 4 * nc};
+    scalar_stage(p, scheme, w2, th2, theta, 1.0 / 3.0, 2.0 / 3.0, th1);
+    or_single_time_step(p, u2, u2 + 2 * nc, u2 + 4 * nc, fx, fy, fz);
+    for (int c = 0; c < 3; c++) axpby(nc, 1.0 / 3.0, u[c], 2.0 / 3.0, u2 + 2 * c * nc, u[c]);
+    memcpy(theta, th1, sizeof(double) * (size_t)nr);
+    free(th1); free(u1); free(u2);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Reductions (Source/min_max_reduction.cu)                                  */
 /* ------------------------------------------------------------------------ */

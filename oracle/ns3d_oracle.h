@@ -86,6 +86,9 @@ void or_weno_advection_spectral(const or_params *p, int scheme,
                                 const double *ux_hat, const double *uy_hat, const double *uz_hat,
                                 double *Qx_hat, double *Qy_hat, double *Qz_hat);        /* WenoAdvection :397-415 */
 
+void or_rk3_ssp_scalar(const or_params *p, double *ux, double *uy, double *uz,
+                       const double *fx, const double *fy, const double *fz, double *theta, int scheme);
+
 /* ---- reductions (min_max_reduction.cu) ---- */
 void or_reduce_minmax(long n, const double *in, double *out2);    /* compute_reduction :199-227 ; out2[0]=min, out2[1]=max */
 double or_reduce_sum(long n, const double *in);                   /* reduction_sum :479-509 */

@@ -182,6 +182,14 @@ class Oracle:
         lib().or_rk3_ssp_uv(ctypes.byref(self.params), *map(_p, u), *map(_p, v), *map(_p, f))
         return u, v
 
+    def rk3_scalar(self, u, f, theta, scheme=5, steps=1):
+        u = [self._c(c).copy() for c in u]
+        f = [self._c(c) for c in f]
+        th = self._r(theta).copy()
+        for _ in range(steps):
+            lib().or_rk3_ssp_scalar(ctypes.byref(self.params), *map(_p, u), *map(_p, f), _p(th), int(scheme))
+        return u, th
+
     def divergence(self, u):
         u = [self._c(c) for c in u]
         d = np.empty(self.cshape, np.complex128)

@@ -234,3 +234,19 @@ def test_large_axes(shape):
     g.RK3_SSP(ud, None)
     zero = [np.zeros(o.cshape, np.complex128)] * 3
     assert rel_l2(to_host(ud), o.rk3(u, zero)) < TOL
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (32, 16, 48)], ids=lambda s: "x".join(map(str, s)))
+def test_rk3_with_passive_scalar(shape):
+    """Velocity RK3 plus a WENO5-advected scalar (config 3's scalar) against the oracle."""
+    from tests.gpu_util import to_dev, to_host
+    print(f"\n[case] scalar grid {shape}", flush=True)
+    g, o = _make(shape, Re=500.0, dt=2e-3)
+    u = _field(shape, seed=23)
+    th = fields.random_scalar(*shape, seed=24)
+    f = fields.kolmogorov_forcing_sin(*shape, n=1)
+    ud, thd = to_dev(u), to_dev([th])[0]
+    g.RK3_SSP_scalar(ud, thd, to_dev(f), steps=2, scheme=5)
+    uo, tho = o.rk3_scalar(u, f, th, scheme=5, steps=2)
+    assert rel_l2(to_host(ud), uo) < TOL
+    assert rel_l2(to_host([thd])[0], tho) < TOL
