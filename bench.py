@@ -108,6 +108,57 @@ def cpu_baseline(N, u0, Re, dt, steps, threads):
                       f"(OpenMP, own mixed-radix FFT, not FFTW), {dt_s:.1f} s"}
 
 
+def run_serving(n_req, rate, seed, precision, max_active, chunk):
+    """Served mixed-size stream (Poisson arrivals, N in {32..192}, Re ~ U(50,500),
+    T ~ U(0.5,5), a quarter J.v requests with T=1) through the continuous-batching
+    server, against one-at-a-time FP64 on the same requests."""
+    import dataclasses
+    import torch
+    from ns3dperiodic_amd import serve
+    t0 = time.time()
+    reqs = serve.poisson_stream(n_req, rate, seed=seed)
+    io = {r.rid: serve.pin_io(*serve.request_inputs(r)) for r in reqs}
+    print(f"[serve] {n_req} requests generated in {time.time() - t0:.1f} s; total steps "
+          f"{sum(r.nsteps for r in reqs)}", file=sys.stderr, flush=True)
+    # warm-up: every (N, kind) once with 1-step horizons, for both paths (contexts, graphs, code objects)
+    warm, wio = [], {}
+    for i, (N, kind) in enumerate(sorted({(r.N, r.kind) for r in reqs})):
+        src = next(r for r in reqs if (r.N, r.kind) == (N, kind))
+        q = dataclasses.replace(src, rid=-1 - i, arrival=0.0, nsteps=chunk + 1, dt=1e-4)
+        warm.append(q)
+        wio[q.rid] = io[src.rid]
+    srv = serve.Server(precision=precision, max_active=max_active, chunk=chunk)
+    srv.run(warm, wio)
+    ctxs = {}
+    serve.one_at_a_time(warm, wio, precision="fp64", ctxs=ctxs)
+    torch.cuda.synchronize()
+    print("[serve] warm-up done", file=sys.stderr, flush=True)
+    res = srv.run(reqs, io)
+    served = {k: [c.copy() for c in r.u] for k, r in res.items()}
+    s_srv = serve.summarize(list(res.values()))
+    print(f"[serve] served {precision}: {s_srv}", file=sys.stderr, flush=True)
+    base = serve.one_at_a_time(reqs, io, precision="fp64", ctxs=ctxs)
+    s_base = serve.summarize(list(base.values()))
+    print(f"[serve] one-at-a-time fp64: {s_base}", file=sys.stderr, flush=True)
+    errs = []
+    for r in reqs:
+        a, b = served[r.rid], base[r.rid].u
+        errs.append(math.sqrt(sum(float(np.sum(np.abs(x - y) ** 2)) for x, y in zip(a, b))
+                              / sum(float(np.sum(np.abs(y) ** 2)) for y in b)))
+    srv.close()
+    for g in ctxs.values():
+        g.close()
+    return {"requests": n_req, "rate_per_s": rate, "seed": seed, "precision": precision,
+            "max_active": max_active, "graph_chunk_steps": chunk,
+            "total_steps": sum(r.nsteps for r in reqs),
+            "solves_per_s": s_srv["solves_per_s"], "p50_latency_s": s_srv["p50_latency_s"],
+            "p99_latency_s": s_srv["p99_latency_s"], "span_s": s_srv["span_s"],
+            "hbm_peak_bytes": srv.ledger.peak,
+            "baseline_fp64_one_at_a_time": s_base,
+            "speedup_solves_per_s": s_srv["solves_per_s"] / s_base["solves_per_s"],
+            "max_rel_l2_vs_fp64": max(errs), "median_rel_l2_vs_fp64": float(np.median(errs))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +170,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests in the served mixed-size stream (0: skip the serving leg)")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-precision", choices=["fp64", "fp32"], default="fp32")
+    ap.add_argument("--serve-active", type=int, default=8, help="resident request slots")
+    ap.add_argument("--serve-chunk", type=int, default=16, help="RK3 steps per captured graph")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (3 times): the reference kernel for HBM counters")
     args = ap.parse_args()
@@ -172,6 +229,11 @@ def main():
         cpu = cpu_baseline(N, u0, Re, dt, args.cpu_steps, threads)
         print(f"[bench] cpu: {cpu['value']:.4f} steps/s", file=sys.stderr, flush=True)
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = run_serving(args.serve_requests, args.serve_rate, args.seed, args.serve_precision,
+                              args.serve_active, args.serve_chunk)
+
     out = {
         "metric": "RK3 steps/s at 256^3 (FP64; FP32-bulk beside it), % of HBM peak",
         "value": prim["steps_per_s"],
@@ -193,6 +255,7 @@ def main():
                           "frac": prim["step_GBps"] / HBM_PEAK_GBS},
         "passes": prim["passes"],
         "finite": prim["finite"],
+        "serving": serving,
     }
     if sec is not None:
         out[f"{sec['precision']}_bulk" if sec["precision"] == "fp32" else sec["precision"]] = {

@@ -130,6 +130,12 @@ int ns3d_weno_advect(ns3d_ctx *ctx, int scheme, const double *ux, const double *
 int ns3d_weno_advection(ns3d_ctx *ctx, int scheme, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
                         ns3d_cplx *Qx, ns3d_cplx *Qy, ns3d_cplx *Qz);             /* WenoAdvection */
 
+/* nsteps of RK3_SSP replayed from a HIP graph captured on the first call (and again
+ * whenever the pointers, nsteps, dt or Re change).  Needs a created stream (not the
+ * null stream).  Same results as ns3d_rk3_ssp_steps; one launch instead of 15*nsteps. */
+int ns3d_rk3_ssp_graph(ns3d_ctx *ctx, int nsteps, ns3d_cplx *ux, ns3d_cplx *uy, ns3d_cplx *uz,
+                       const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz);
+
 /* RK3_SSP of the velocity plus a passive scalar theta (physical, double [Nx][Ny][Nz]) advected
  * with WENO (scheme 3/5) by each stage's 2/3-masked physical velocity, with the same SSP
  * weights: theta_t = -(u.grad) theta.  No reference counterpart as a whole: it couples

@@ -186,10 +186,21 @@ template <int N, int TPP, int... R> struct PlanRegs<N, TPP, Seq<R...>> {
 // (pencils are consecutive z-columns: conflict-free when lanes are p-fastest).
 // PencilMajor: p*(N+PAD)+idx (pencils contiguous; lanes t-fastest).
 template <int P> struct IndexMajor {
-    __device__ __forceinline__ static int at(int p, int idx) { return idx * P + p; }
+    template <int NS = 0> __device__ __forceinline__ static int at(int p, int idx) { return idx * P + p; }
 };
 template <int N, int PAD> struct PencilMajor {
-    __device__ __forceinline__ static int at(int p, int idx) { return p * (N + PAD) + idx; }
+    template <int NS = 0> __device__ __forceinline__ static int at(int p, int idx) { return p * (N + PAD) + idx; }
+};
+// PencilMajor with a bank swizzle.  The exchange after a pass with stride Ns
+// writes runs of Ns consecutive elements separated by Ns*R; XOR-ing the low
+// log2(G) index bits with (idx/G)*Ns spreads such writes over all banks of a
+// G-element group, and stays a permutation within each aligned group, so the
+// next pass's contiguous reads remain conflict-free.  NS = 0: no swizzle.
+template <int N, int PAD, int G> struct PencilMajorSw {
+    template <int NS = 0> __device__ __forceinline__ static int at(int p, int idx) {
+        if constexpr (NS > 0 && NS < G && N % G == 0) idx ^= ((idx / G) * NS) & (G - 1);
+        return p * (N + PAD) + idx;
+    }
 };
 
 template <typename T, int N, int TPP, int DIR, typename S, typename Addr>
@@ -239,7 +250,7 @@ struct PencilFFT {
                 if (NB % TPP == 0 || jj < NB) {
                     const int base = (jj / Ns) * Ns * R + (jj % Ns);
 #pragma unroll
-                    for (int r = 0; r < R; r++) lds[Addr::at(p, base + r * Ns)] = v[q * R + r];
+                    for (int r = 0; r < R; r++) lds[Addr::template at<Ns>(p, base + r * Ns)] = v[q * R + r];
                 }
             }
             __syncthreads();
@@ -251,7 +262,7 @@ struct PencilFFT {
                 const int jj = t + q * TPP;
                 if (NB2 % TPP == 0 || jj < NB2) {
 #pragma unroll
-                    for (int r = 0; r < R2; r++) v[q * R2 + r] = lds[Addr::at(p, jj + r * NB2)];
+                    for (int r = 0; r < R2; r++) v[q * R2 + r] = lds[Addr::template at<Ns>(p, jj + r * NB2)];
                 }
             }
             __syncthreads();

@@ -12,6 +12,8 @@
 // Real-space fields never touch HBM.  NF=2 evaluates B(u,v) of the J.v path
 // (calculate_convolution_2p3_UV, :500-525) with the same passes.
 #pragma once
+#include <type_traits>
+
 #include "fft_core.hpp"
 
 namespace ns3d {
@@ -204,7 +206,12 @@ template <typename TB, int NZ, int P, bool TRUNC = false>
 struct ZPencil {
     static constexpr int TPP = ZPlan<NZ>::TPP;
     static constexpr int PAD = (sizeof(TB) == 8) ? 1 : 2;
-    using LA = PencilMajor<NZ, PAD>;
+#ifndef NS3D_ZSWZ
+#define NS3D_ZSWZ 16
+#endif
+    // swizzle group in elements: NS3D_ZSWZ (FP64) or twice that (FP32); 0 disables
+    using LA = typename std::conditional<NS3D_ZSWZ == 0, PencilMajor<NZ, PAD>,
+                                         PencilMajorSw<NZ, PAD, (NS3D_ZSWZ == 0 ? 1 : NS3D_ZSWZ) * (sizeof(TB) == 8 ? 1 : 2)>>::type;
     using FI = PencilFFT<TB, NZ, TPP, +1, typename ZPlan<NZ>::R, LA>;
     using FF = PencilFFT<TB, NZ, TPP, -1, typename SeqRev<typename ZPlan<NZ>::R>::type, LA>;
     static_assert(FI::RL == FF::R0 && FI::BL == FF::B0, "mirror plans must share the product layout");

@@ -98,6 +98,12 @@ struct ns3d_ctx {
     int timesteps_period = 1;
     double tau = 1.0;
     zd* kry = nullptr;  // 3 fields for J.v unpacking
+    // a captured graph of nsteps RK3 steps for fixed state/forcing pointers, dt and Re
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    const void* graph_key[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int graph_steps = -1;
+    double graph_dt = 0.0, graph_Re = 0.0;
     // passive scalar: physical velocity written by the z pass (bulk precision, 3 real fields)
     void* uphys = nullptr;
     void* uphys_out = nullptr;  // non-null only while a scalar step wants it
@@ -250,8 +256,18 @@ template <typename TB> const cplx<TB>* twp(const ns3d_ctx* c, int axis) {
 
 size_t bulk_bytes(const ns3d_ctx* c) { return c->precision == NS3D_FP64 ? sizeof(zd) : sizeof(cplx<float>); }
 
+// Forget a captured RK3 graph (its nodes hold scratch pointers, table values and the stream).
+void drop_graph(ns3d_ctx* c) {
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->graph_steps = -1;
+}
+
 int ensure_scratch(ns3d_ctx* c, int fields) {
     if (c->scratch_fields >= fields) return NS3D_OK;
+    drop_graph(c);
     if (c->scratch) {
         hipStreamSynchronize(c->stream);
         hipFree(c->scratch);
@@ -694,6 +710,8 @@ int ns3d_destroy(ns3d_ctx* c) {
     hipFree(c->theta_stage);
     if (c->red_host) hipHostFree(c->red_host);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
     delete c;
     return NS3D_OK;
 }
@@ -919,6 +937,49 @@ int ns3d_rk3_ssp_scalar(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, n
         int st = rk3_scalar_one(c, u, f, theta, scheme);
         if (st) return st;
     }
+    return NS3D_OK;
+}
+
+int ns3d_rk3_ssp_graph(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns3d_cplx* uz, const ns3d_cplx* fx,
+                       const ns3d_cplx* fy, const ns3d_cplx* fz) {
+    if (!c || !ux || !uy || !uz || nsteps < 1) return NS3D_ERR_ARG;
+    if (c->stream == nullptr) return NS3D_ERR_ARG;  // capture needs a created stream
+    const void* key[6] = {ux, uy, uz, fx, fy, fz};
+    bool same = c->graph_exec && c->graph_steps == nsteps && c->graph_dt == c->dt && c->graph_Re == c->Re;
+    for (int i = 0; same && i < 6; i++) same = key[i] == c->graph_key[i];
+    if (!same) {
+        drop_graph(c);
+        // every buffer the step uses is allocated before capture
+        int st = ensure_scratch(c, 15);
+        if (!st) st = ensure_fp64(c, &c->stage, 9);
+        if (st) return st;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const bool prof = c->prof_on;
+        c->prof_on = false;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        zd* u[3] = {(zd*)ux, (zd*)uy, (zd*)uz};
+        const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
+        for (int i = 0; i < nsteps && !st; i++) st = rk3_one(c, u, f);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        c->prof_on = prof;
+        if (st || e != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return st ? st : NS3D_ERR_HIP;
+        }
+        c->graph = g;
+        if (hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            hipGraphDestroy(g);
+            c->graph = nullptr;
+            c->graph_exec = nullptr;
+            return NS3D_ERR_HIP;
+        }
+        for (int i = 0; i < 6; i++) c->graph_key[i] = key[i];
+        c->graph_steps = nsteps;
+        c->graph_dt = c->dt;
+        c->graph_Re = c->Re;
+    }
+    HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
     return NS3D_OK;
 }
 

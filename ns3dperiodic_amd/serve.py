@@ -1,0 +1,386 @@
+"""Continuous-batching request server for mixed-size solves on one GPU.
+
+A request is a whole solve on its own grid: ``integrate`` advances a seeded
+Kolmogorov-forced field to its horizon T with RK3_SSP
+(Source/RK_time_step.cu:310-327); ``jv`` carries a perturbation along the
+trajectory with the tangent map RK3_SSP_UV (Source/RK_time_step.cu:342-355),
+i.e. v(T) = dPhi_T(u0) v0, the product the reference's Krylov drivers build
+from finite differences of the time-T map.
+
+The server keeps every admitted request's state resident in HBM and admits
+new requests while others run, as soon as a slot and the HBM budget allow.
+Each slot owns a context and a stream; its steps are issued in chunks of K
+RK3 steps replayed from a captured HIP graph, at most two chunks in flight
+per slot, so the host loop keeps every active stream fed and the device
+overlaps the kernels of all active requests.  A request's arithmetic is the
+same kernels in the same order whether it runs alone or in a mix, so its
+result is bitwise the same either way.
+
+Scheduling (``Ledger``, ``poisson_stream``, ``choose_dt``) is plain Python
+without GPU calls; ``Server`` and ``one_at_a_time`` drive the HIP library.
+"""
+import collections
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+from . import fields
+
+GRID_SIZES = (32, 48, 64, 96, 128, 192)
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    kind: str          # "integrate" | "jv"
+    N: int             # cubic grid N^3 on [0, 2 pi]^3
+    Re: float
+    T: float           # horizon
+    seed: int
+    arrival: float     # seconds after the stream starts
+    nsteps: int = 0
+    dt: float = 0.0
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    admitted: float
+    done: float
+    latency: float
+    u: list            # final state, host complex128 [N, N, N/2+1] x 3
+    v: list = None     # final perturbation (jv)
+
+
+def choose_dt(u_hat, T, N, cfl=0.2, L=2 * math.pi):
+    """dt = cfl * min dx / max|u0|, shrunk so that T/dt is an integer.
+    Returns (nsteps, dt).  max|u0| is the largest velocity component in
+    physical space (as calc_dt uses, Source/cuda_supp.cu:170-238)."""
+    umax = max(float(np.max(np.abs(np.fft.irfftn(c, s=(N, N, N), axes=(0, 1, 2))))) for c in u_hat)
+    dt0 = cfl * (L / N) / max(umax, 1e-30)
+    n = max(1, int(math.ceil(T / dt0 - 1e-12)))
+    return n, T / n
+
+
+def poisson_stream(n, rate, seed=0, sizes=GRID_SIZES, re_range=(50.0, 500.0), t_range=(0.5, 5.0),
+                   jv_fraction=0.25, jv_T=1.0):
+    """Seeded request stream: exponential inter-arrival times at `rate` per
+    second, N uniform over `sizes`, Re ~ U(re_range), T ~ U(t_range) for
+    integrate requests and T = jv_T for J.v requests."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += float(rng.exponential(1.0 / rate)) if rate > 0 else 0.0
+        kind = "jv" if rng.random() < jv_fraction else "integrate"
+        N = int(sizes[int(rng.integers(len(sizes)))])
+        Re = float(rng.uniform(*re_range))
+        T = float(rng.uniform(*t_range)) if kind == "integrate" else float(jv_T)
+        out.append(Request(i, kind, N, Re, T, seed=1000 * seed + i, arrival=t))
+    return out
+
+
+def request_inputs(req, energy=0.5):
+    """Host inputs of a request: config-2 style seeded solenoidal field
+    (E(k) ~ k^4 exp(-2 (k/4)^2), energy 0.5), the forcing f_x = sin 4y, and for
+    J.v a unit-norm solenoidal perturbation.  Fills req.nsteps / req.dt."""
+    N = req.N
+    u = fields.random_solenoidal(N, seed=req.seed, energy=energy)
+    fx = fields.kolmogorov_forcing_sin(N, n=4)[0]
+    v = None
+    if req.kind == "jv":
+        v = fields.random_solenoidal(N, seed=req.seed + 7919, energy=1.0)
+        nrm = math.sqrt(sum(float(np.vdot(c, c).real) for c in v))
+        v = [c / nrm for c in v]
+    req.nsteps, req.dt = choose_dt(u, req.T, N)
+    return u, fx, v
+
+
+def field_bytes(N, precision):
+    """HBM held by an integrate request on an N^3 grid: 3 state fields and the
+    forcing x-component (FP64), 9 FP64 RK stage fields and 15 scratch fields
+    in the bulk precision."""
+    nc = N * N * (N // 2 + 1)
+    bulk = 16 if precision == "fp64" else 8
+    return nc * (16 * (3 + 1 + 9) + bulk * 15)
+
+
+def request_bytes(req, precision):
+    """field_bytes plus, for J.v, the perturbation (3 FP64 fields), the
+    tangent map's 6 FP64 stage fields and 15 more scratch fields (B(u,v))."""
+    nc = req.N * req.N * (req.N // 2 + 1)
+    b = field_bytes(req.N, precision)
+    if req.kind == "jv":
+        bulk = 16 if precision == "fp64" else 8
+        b += nc * (16 * (3 + 6) + bulk * 15)
+    return b
+
+
+class Ledger:
+    """HBM budget and slot accounting for admission (no GPU calls)."""
+
+    def __init__(self, budget_bytes, max_active):
+        self.budget = int(budget_bytes)
+        self.max_active = int(max_active)
+        self.used = 0
+        self.active = 0
+        self.peak = 0
+        self.peak_active = 0
+
+    def fits(self, nbytes):
+        return self.active < self.max_active and self.used + nbytes <= self.budget
+
+    def admit(self, nbytes):
+        if not self.fits(nbytes):
+            raise RuntimeError("admission beyond budget or slot limit")
+        self.used += nbytes
+        self.active += 1
+        self.peak = max(self.peak, self.used)
+        self.peak_active = max(self.peak_active, self.active)
+
+    def release(self, nbytes):
+        self.used -= nbytes
+        self.active -= 1
+        assert self.used >= 0 and self.active >= 0
+
+
+def admit_order(queue, ledger, cost, max_overtakes=2):
+    """Requests of the FIFO `queue` to admit now.  Requests are taken in
+    order while slots and budget allow; a request that does not fit may be
+    overtaken by smaller ones behind it, at most `max_overtakes` times, after
+    which nothing behind it is admitted before it (no starvation)."""
+    picked = []
+    waiting = []                  # earlier requests that did not fit
+    used, active = ledger.used, ledger.active
+    for r in list(queue):
+        if active >= ledger.max_active:
+            break
+        c = cost(r)
+        if used + c > ledger.budget:
+            waiting.append(r)
+            continue
+        if any(getattr(w, "overtaken", 0) >= max_overtakes for w in waiting):
+            break
+        for w in waiting:
+            w.overtaken = getattr(w, "overtaken", 0) + 1
+        picked.append(r)
+        used += c
+        active += 1
+    return picked
+
+
+def summarize(results, t_first_arrival=0.0):
+    lat = np.array([r.latency for r in results])
+    span = max(r.done for r in results) - t_first_arrival
+    return {"solves": len(results), "span_s": span, "solves_per_s": len(results) / span,
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99))}
+
+
+# ---------------------------------------------------------------------------- GPU side
+class _Slot:
+    """A context, a stream and resident buffers for one request at a time;
+    reused by later requests of the same size and kind."""
+
+    def __init__(self, N, kind, precision, device, chunk, use_graphs):
+        import torch
+        from .solver import NS3D
+        self.N, self.kind = N, kind
+        self.bytes = request_bytes(Request(-1, kind, N, 1.0, 1.0, 0, 0.0), precision)
+        self.stream = torch.cuda.Stream(device=device)
+        self.g = NS3D(N, precision=precision, stream=self.stream, device=device)
+        cs = (N, N, N // 2 + 1)
+        dev = torch.device("cuda", device)
+        self.u = [torch.empty(cs, dtype=torch.complex128, device=dev) for _ in range(3)]
+        self.v = [torch.empty(cs, dtype=torch.complex128, device=dev) for _ in range(3)] if kind == "jv" else None
+        self.f = [torch.empty(cs, dtype=torch.complex128, device=dev), None, None]
+        self.chunk = chunk
+        self.use_graphs = use_graphs
+        self.req = None
+        self.done_ev = None
+
+    def load(self, req, io):
+        import torch
+        self.req, self.io = req, io
+        self.g.set_params(req.dt, req.Re)
+        with torch.cuda.stream(self.stream):
+            for d, h in zip(self.u, io["u"]):
+                d.copy_(h, non_blocking=True)
+            self.f[0].copy_(io["f"], non_blocking=True)
+            if req.kind == "jv":
+                for d, h in zip(self.v, io["v"]):
+                    d.copy_(h, non_blocking=True)
+        self.left = req.nsteps
+        self.inflight = collections.deque()
+
+    def issue(self):
+        """Enqueue the next chunk of at most K steps; False once all are issued."""
+        import torch
+        if self.left <= 0:
+            return False
+        k = min(self.left, self.chunk)
+        if self.req.kind == "jv":
+            for _ in range(k):
+                self.g.RK3_SSP_UV(self.u, self.v, self.f)
+        elif self.use_graphs and k == self.chunk:
+            self.g.RK3_SSP_graph(self.u, self.f, steps=k)
+        else:
+            self.g.RK3_SSP(self.u, self.f, steps=k)
+        self.left -= k
+        ev = torch.cuda.Event()
+        ev.record(self.stream)
+        self.inflight.append(ev)
+        return True
+
+    def start_download(self):
+        import torch
+        with torch.cuda.stream(self.stream):
+            for h, d in zip(self.io["out_u"], self.u):
+                h.copy_(d, non_blocking=True)
+            if self.req.kind == "jv":
+                for h, d in zip(self.io["out_v"], self.v):
+                    h.copy_(d, non_blocking=True)
+        self.done_ev = torch.cuda.Event()
+        self.done_ev.record(self.stream)
+
+
+class Server:
+    """Continuous-batching server: up to `max_active` resident requests, each
+    on its own slot (context + stream), admitted by HBM budget."""
+
+    def __init__(self, precision="fp32", max_active=8, budget_bytes=64 << 30, chunk=16, use_graphs=True,
+                 device=0, inflight=2):
+        self.precision = precision
+        self.ledger = Ledger(budget_bytes, max_active)
+        self.chunk = chunk
+        self.use_graphs = use_graphs
+        self.device = device
+        self.inflight = inflight
+        self.idle = collections.defaultdict(list)     # (N, kind) -> idle slots (context, graph, buffers kept)
+        self.idle_bytes = 0
+
+    def _slot(self, r):
+        key = (r.N, r.kind)
+        if self.idle[key]:
+            s = self.idle[key].pop()
+            self.idle_bytes -= s.bytes
+            return s
+        return _Slot(r.N, r.kind, self.precision, self.device, self.chunk, self.use_graphs)
+
+    def _make_room(self, need):
+        # drop idle slots (largest first) while the budget is short
+        for key in sorted(self.idle, key=lambda k: -k[0]):
+            while self.idle[key] and self.ledger.used + self.idle_bytes + need > self.ledger.budget:
+                s = self.idle[key].pop()
+                s.g.close()
+                self.idle_bytes -= s.bytes
+
+    def run(self, requests, io, clock=time.perf_counter):
+        """Serve `requests`; io[rid] holds the request's pinned host buffers
+        (see pin_io).  Returns {rid: Result}; results stay in io[rid]."""
+        import torch
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        queue = collections.deque()
+        active = []
+        results = {}
+        cost = lambda r: request_bytes(r, self.precision)   # noqa: E731
+        for r in requests:
+            if cost(r) > self.ledger.budget:
+                raise ValueError(f"request {r.rid} needs {cost(r)} bytes, more than the whole budget")
+            if r.nsteps < 1:
+                raise ValueError(f"request {r.rid} has no steps (request_inputs sets them)")
+        t0 = clock()
+        while pending or queue or active:
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                queue.append(pending.popleft())
+            for r in admit_order(queue, self.ledger, cost):
+                self._make_room(cost(r))
+                queue.remove(r)
+                self.ledger.admit(cost(r))
+                s = self._slot(r)
+                s.load(r, io[r.rid])
+                s.admitted = clock() - t0
+                active.append(s)
+            progressed = False
+            for s in list(active):
+                while s.inflight and s.inflight[0].query():
+                    s.inflight.popleft()
+                while len(s.inflight) < self.inflight and s.issue():
+                    progressed = True
+                if s.left == 0 and not s.inflight and s.done_ev is None:
+                    s.start_download()
+                if s.done_ev is not None and s.done_ev.query():
+                    t = clock() - t0
+                    r = s.req
+                    out_v = [c.numpy() for c in s.io["out_v"]] if r.kind == "jv" else None
+                    results[r.rid] = Result(r.rid, s.admitted, t, t - r.arrival,
+                                            [c.numpy() for c in s.io["out_u"]], out_v)
+                    s.done_ev = None
+                    s.req = s.io = None
+                    active.remove(s)
+                    self.ledger.release(cost(r))
+                    self.idle[(r.N, r.kind)].append(s)
+                    self.idle_bytes += s.bytes
+                    progressed = True
+            if not progressed:
+                time.sleep(20e-6)
+        torch.cuda.synchronize(self.device)
+        return results
+
+    def close(self):
+        for lst in self.idle.values():
+            for s in lst:
+                s.g.close()
+        self.idle.clear()
+
+
+def pin_io(u, fx, v):
+    """Pinned host buffers of one request: its inputs (what a client hands
+    over) and room for its outputs."""
+    import torch
+    pin = lambda a: torch.from_numpy(np.ascontiguousarray(a)).pin_memory()   # noqa: E731
+    io = {"u": [pin(c) for c in u], "f": pin(fx), "v": [pin(c) for c in v] if v is not None else None}
+    io["out_u"] = [torch.empty_like(c).pin_memory() for c in io["u"]]
+    io["out_v"] = [torch.empty_like(c).pin_memory() for c in io["v"]] if v is not None else None
+    return io
+
+
+def one_at_a_time(requests, io, precision="fp64", device=0, clock=time.perf_counter, ctxs=None):
+    """Baseline: requests in arrival order, one at a time, eager RK3 steps on
+    one context per grid size (contexts reused; pass `ctxs` to keep them
+    across calls), upload and download included."""
+    import torch
+    from .solver import NS3D
+    own = ctxs is None
+    ctxs = {} if own else ctxs
+    results = {}
+    t0 = clock()
+    dev = torch.device("cuda", device)
+    for r in sorted(requests, key=lambda q: q.arrival):
+        while clock() - t0 < r.arrival:
+            time.sleep(50e-6)
+        adm = clock() - t0
+        if r.N not in ctxs:
+            ctxs[r.N] = NS3D(r.N, precision=precision, device=device)
+        g = ctxs[r.N]
+        g.set_params(r.dt, r.Re)
+        inp = io[r.rid]
+        u = [h.to(dev) for h in inp["u"]]
+        f = [inp["f"].to(dev), None, None]
+        if r.kind == "jv":
+            v = [h.to(dev) for h in inp["v"]]
+            for _ in range(r.nsteps):
+                g.RK3_SSP_UV(u, v, f)
+        else:
+            v = None
+            g.RK3_SSP(u, f, steps=r.nsteps)
+        out_u = [c.cpu().numpy() for c in u]
+        out_v = [c.cpu().numpy() for c in v] if v is not None else None
+        t = clock() - t0
+        results[r.rid] = Result(r.rid, adm, t, t - r.arrival, out_u, out_v)
+    if own:
+        for g in ctxs.values():
+            g.close()
+    return results

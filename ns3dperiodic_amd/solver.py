@@ -172,6 +172,11 @@ class NS3D:
         check(self.lib.ns3d_rk3_ssp_steps(self.ctx, steps, *self._c3(u), *self._f3(f)), "ns3d_rk3_ssp_steps")
         return u
 
+    def RK3_SSP_graph(self, u, f=None, steps=1):
+        """steps RK3 steps replayed from a captured HIP graph (needs a non-default stream)."""
+        check(self.lib.ns3d_rk3_ssp_graph(self.ctx, steps, *self._c3(u), *self._f3(f)), "ns3d_rk3_ssp_graph")
+        return u
+
     def RK3_SSP_scalar(self, u, theta, f=None, steps=1, scheme=5):
         """RK3 of the velocity plus a WENO-advected passive scalar theta (physical float64)."""
         assert theta.dtype == torch.float64 and tuple(theta.shape) == self.rshape
