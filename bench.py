@@ -140,9 +140,23 @@ def run_serving(n_req, rate, seed, precision, max_active, chunk):
     base = serve.one_at_a_time(reqs, io, precision="fp64", ctxs=ctxs)
     s_base = serve.summarize(list(base.values()))
     print(f"[serve] one-at-a-time fp64: {s_base}", file=sys.stderr, flush=True)
-    errs = []
+    # per grid size: requests, steps and one-at-a-time FP64 service seconds (done - start)
+    by_n = {}
+    for r in reqs:
+        d = by_n.setdefault(r.N, {"requests": 0, "steps": 0, "fp64_service_s": 0.0})
+        d["requests"] += 1
+        d["steps"] += r.nsteps
+        d["fp64_service_s"] += base[r.rid].done - base[r.rid].admitted
+    for d in by_n.values():
+        d["fp64_ms_per_step"] = 1e3 * d["fp64_service_s"] / max(d["steps"], 1)
+    errs, nonfinite = [], []
     for r in reqs:
         a, b = served[r.rid], base[r.rid].u
+        if not all(np.isfinite(x).all() for x in a + b):
+            nonfinite.append(r.rid)
+            print(f"[serve] request {r.rid} ({r.kind} N={r.N} Re={r.Re:.0f} T={r.T:.2f} steps={r.nsteps}) "
+                  f"is not finite", file=sys.stderr, flush=True)
+            continue
         errs.append(math.sqrt(sum(float(np.sum(np.abs(x - y) ** 2)) for x, y in zip(a, b))
                               / sum(float(np.sum(np.abs(y) ** 2)) for y in b)))
     srv.close()
@@ -156,7 +170,9 @@ def run_serving(n_req, rate, seed, precision, max_active, chunk):
             "hbm_peak_bytes": srv.ledger.peak,
             "baseline_fp64_one_at_a_time": s_base,
             "speedup_solves_per_s": s_srv["solves_per_s"] / s_base["solves_per_s"],
-            "max_rel_l2_vs_fp64": max(errs), "median_rel_l2_vs_fp64": float(np.median(errs))}
+            "max_rel_l2_vs_fp64": max(errs) if errs else None,
+            "median_rel_l2_vs_fp64": float(np.median(errs)) if errs else None, "nonfinite": nonfinite,
+            "by_grid": {str(k): v for k, v in sorted(by_n.items())}}
 
 
 def main():
@@ -165,8 +181,9 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--N", type=int, default=256)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--secondary", choices=["fp64", "fp32", "none"], default="fp32")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "fp32s"], default="fp64")
+    ap.add_argument("--secondary", default="fp32,fp32s",
+                    help="comma list of further precisions to time beside the primary (fp64, fp32, fp32s) or none")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
@@ -213,10 +230,12 @@ def main():
 
     prim = run_gpu(N, args.precision, args.steps, args.warmup, u0, Re, dt)
     print(f"[bench] {args.precision}: {prim['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
-    sec = None
-    if args.secondary != "none" and args.secondary != args.precision:
-        sec = run_gpu(N, args.secondary, args.steps, args.warmup, u0, Re, dt)
-        print(f"[bench] {args.secondary}: {sec['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+    secs = []
+    for p in [] if args.secondary == "none" else args.secondary.split(","):
+        if p == args.precision:
+            continue
+        secs.append(run_gpu(N, p, args.steps, args.warmup, u0, Re, dt))
+        print(f"[bench] {p}: {secs[-1]['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
 
     dom = max(prim["passes"], key=lambda p: p["total_ms"])
     roofline = {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
@@ -245,7 +264,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 bulk (f64 state)",
+        "dtype": {"fp64": "f64", "fp32": "f32 bulk (f64 state)", "fp32s": "f32 (f32 state)"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"decaying turbulence {N}^3, nu=2e-3, k0=4 spectrum, reference mode",
                    "N": N, "Re": Re, "dt": dt, "global_batch": 1, "parallelism": "single GPU"},
@@ -257,8 +276,9 @@ def main():
         "finite": prim["finite"],
         "serving": serving,
     }
-    if sec is not None:
-        out[f"{sec['precision']}_bulk" if sec["precision"] == "fp32" else sec["precision"]] = {
+    names = {"fp64": "fp64", "fp32": "fp32_bulk", "fp32s": "fp32_state"}
+    for sec in secs:
+        out[names[sec["precision"]]] = {
             "steps_per_s": sec["steps_per_s"], "ms_per_step": sec["ms_per_step"],
             "speedup_vs_primary": sec["steps_per_s"] / prim["steps_per_s"],
             "step_roofline_frac": sec["step_GBps"] / HBM_PEAK_GBS,

@@ -38,20 +38,25 @@ typedef struct { double x, y; } ns3d_cplx;
 #define NS3D_ERR_NONFINITE (-5)    /* NaN/Inf detected by a checking call */
 #define NS3D_ERR_PRECISION (-6)    /* operation not offered at this precision */
 
-/* precision of the transform / product bulk; state arrays are always FP64 */
+/* precision of the transform / product bulk; state arrays at the interface are always FP64 */
 #define NS3D_FP64 0
-#define NS3D_FP32_BULK 1
+#define NS3D_FP32_BULK 1   /* FP32 transforms and products; state and epilogue FP64 */
+#define NS3D_FP32_STATE 2  /* as FP32_BULK, and the RK3 loops (ns3d_rk3_ssp*, graph) keep the state and
+                              stages in FP32, converting the caller's FP64 state once per call */
 
 /* behaviour flags (reference quirks are ON in reference mode) */
 #define NS3D_FLAG_NO_IPQ 0x1u      /* disable zeroing Re(Q^) and Q^(0) (advection_2_3.cu:35-70) */
 #define NS3D_FLAG_WENO_ZFIX 0x2u   /* use l-2 in flux_z_weno_R (advection_WENO.cu:288 reads l-1) */
+#define NS3D_FLAG_MASK_NYQUIST 0x4u /* 2/3 mask relative to each axis's Nyquist wavenumber (alias-free);
+                                       the reference normalises by (Nx, Ny, Mz) (math_support.cu:471-519),
+                                       which leaves the x and y axes untruncated */
 
 typedef struct {
     int Nx, Ny, Nz;        /* physical grid, each in {8,16,32,48,64,96,128,192,256,512} */
     double Lx, Ly, Lz;     /* box lengths (reference: 2*pi/alpha, 2*pi, 2*pi/beta, NS3D_periodic.cu:969-979) */
     double Re;             /* Reynolds number, nu = 1/Re */
     double dt;             /* time step */
-    int precision;         /* NS3D_FP64 or NS3D_FP32_BULK */
+    int precision;         /* NS3D_FP64, NS3D_FP32_BULK or NS3D_FP32_STATE */
     unsigned flags;        /* NS3D_FLAG_* */
     void *stream;          /* hipStream_t; NULL = default stream */
     int device;            /* HIP device ordinal; -1 = current device */

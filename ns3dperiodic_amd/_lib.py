@@ -12,8 +12,10 @@ LIB_PATH = os.path.join(HERE, "libns3d_hip.so")
 NS3D_OK = 0
 NS3D_FP64 = 0
 NS3D_FP32_BULK = 1
+NS3D_FP32_STATE = 2
 NS3D_FLAG_NO_IPQ = 0x1
 NS3D_FLAG_WENO_ZFIX = 0x2
+NS3D_FLAG_MASK_NYQUIST = 0x4
 UPD_STAGE, UPD_UV_RHS, UPD_UV, UPD_IUV, UPD_RHS = 0, 1, 2, 3, 4
 
 

@@ -42,9 +42,10 @@ template <int N, int E> using XYPlan = typename XYSel<N, E>::type;
 // consecutive z indices l; lanes p-fastest so each row load is contiguous.
 // grid.y enumerates the fields; with OS = 1 each block produces one output
 // (w_b or dx w_b) and the input is read by two blocks.
-template <typename TB, int NX, int P, int NF, int E, int OS>
+// TS: scalar type of the state w (double, or float for the FP32-state RK3 loop).
+template <typename TB, int NX, int P, int NF, int E, int OS, typename TS = double>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
-k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int nlt, int nb,
+k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int nlt, int nb,
              SpecTables tab, const cplx<TB>* __restrict__ twg) {
     constexpr int TPP = XYPlan<NX, E>::TPP;
     using F = PencilFFT<TB, NX, TPP, +1, typename XYPlan<NX, E>::R, IndexMajor<P>>;
@@ -64,7 +65,7 @@ k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, lon
     const long base = (long)k * Mz + l;
     const int f = OS ? (int)blockIdx.y >> 1 : (int)blockIdx.y;
     const int which = OS ? (int)blockIdx.y & 1 : 0;  // OS: 0 -> w_b, 1 -> dx w_b
-    const cplx<double>* __restrict__ src = w.p[f];
+    const cplx<TS>* __restrict__ src = w.p[f];
     cplx<TB> v[F::E], d[OS ? 1 : F::E];
 #pragma unroll
     for (int q = 0; q < F::B0; q++) {
@@ -75,7 +76,7 @@ k_x_inv_grad(PtrArr<const cplx<double>, 3 * NF> w, cplx<TB>* __restrict__ A, lon
             double kx = 0.0;
             if (F::in_active(t, q)) {
                 kx = tab.kx[j];
-                if (keep && mask_keep(tab, j, k, l)) s = src[base + j * S];
+                if (keep && mask_keep(tab, j, k, l)) s = cvt<double>(src[base + j * S]);
             }
             const cplx<TB> dv = mk<TB>((TB)(-s.y * kx), (TB)(s.x * kx));
             if constexpr (OS) {
@@ -130,6 +131,24 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
     const int f = OS ? (int)blockIdx.y / 3 : (int)blockIdx.y;
     const int which = OS ? (int)blockIdx.y % 3 : -1;  // OS: 0 -> w_b, 1 -> dy w_b, 2 -> dx w_b
     const int s_ = f / 3, b = f % 3;
+#ifndef NS3D_HOIST
+#define NS3D_HOIST 1
+#endif
+    // x-derivative input, loaded before the first transforms when both are done here
+    cplx<TB> v2[(!OS && NS3D_HOIST) ? F::E : 1];
+    if constexpr (!OS && NS3D_HOIST) {
+        const cplx<TB>* __restrict__ src = A + (long)(s_ * 6 + 3 + b) * nc;
+#pragma unroll
+        for (int q = 0; q < F::B0; q++) {
+#pragma unroll
+            for (int r = 0; r < F::R0; r++) {
+                const int kk = F::in_index(t, q, r);
+                cplx<TB> s = mk<TB>(0, 0);
+                if (F::in_active(t, q) && lv && column_kept(tab, kk, l)) s = src[base + kk * S];
+                v2[q * F::R0 + r] = s;
+            }
+        }
+    }
     // velocity component: produce w_b and dy w_b
     if (!OS || which < 2) {
         const cplx<TB>* __restrict__ src = A + (long)(s_ * 6 + b) * nc;
@@ -173,16 +192,21 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
     }
     // x-derivative component: plain y-inverse
     if (!OS || which == 2) {
-        const cplx<TB>* __restrict__ src = A + (long)(s_ * 6 + 3 + b) * nc;
         cplx<TB> v[F::E];
+        if constexpr (!OS && NS3D_HOIST) {
 #pragma unroll
-        for (int q = 0; q < F::B0; q++) {
+            for (int e = 0; e < F::E; e++) v[e] = v2[e];
+        } else {
+            const cplx<TB>* __restrict__ src = A + (long)(s_ * 6 + 3 + b) * nc;
 #pragma unroll
-            for (int r = 0; r < F::R0; r++) {
-                const int kk = F::in_index(t, q, r);
-                cplx<TB> s = mk<TB>(0, 0);
-                if (F::in_active(t, q) && lv && column_kept(tab, kk, l)) s = src[base + kk * S];
-                v[q * F::R0 + r] = s;
+            for (int q = 0; q < F::B0; q++) {
+#pragma unroll
+                for (int r = 0; r < F::R0; r++) {
+                    const int kk = F::in_index(t, q, r);
+                    cplx<TB> s = mk<TB>(0, 0);
+                    if (F::in_active(t, q) && lv && column_kept(tab, kk, l)) s = src[base + kk * S];
+                    v[q * F::R0 + r] = s;
+                }
             }
         }
         F::run(v, lds, tw, t, p);
@@ -209,9 +233,12 @@ struct ZPencil {
 #ifndef NS3D_ZSWZ
 #define NS3D_ZSWZ 16
 #endif
-    // swizzle group in elements: NS3D_ZSWZ (FP64) or twice that (FP32); 0 disables
-    using LA = typename std::conditional<NS3D_ZSWZ == 0, PencilMajor<NZ, PAD>,
-                                         PencilMajorSw<NZ, PAD, (NS3D_ZSWZ == 0 ? 1 : NS3D_ZSWZ) * (sizeof(TB) == 8 ? 1 : 2)>>::type;
+#ifndef NS3D_ZSWZ32
+#define NS3D_ZSWZ32 0
+#endif
+    // LDS swizzle group in elements (0: none); measured: it pays for FP64 only
+    static constexpr int SWG = sizeof(TB) == 8 ? NS3D_ZSWZ : NS3D_ZSWZ32;
+    using LA = typename std::conditional<SWG == 0, PencilMajor<NZ, PAD>, PencilMajorSw<NZ, PAD, (SWG == 0 ? 1 : SWG)>>::type;
     using FI = PencilFFT<TB, NZ, TPP, +1, typename ZPlan<NZ>::R, LA>;
     using FF = PencilFFT<TB, NZ, TPP, -1, typename SeqRev<typename ZPlan<NZ>::R>::type, LA>;
     static_assert(FI::RL == FF::R0 && FI::BL == FF::B0, "mirror plans must share the product layout");
@@ -247,6 +274,65 @@ struct ZPencil {
                     b.y = -b.y;
                 }
                 v[q * FI::R0 + r] = mk<TB>(a.x - b.y, a.y + b.x);
+            }
+        }
+    }
+
+    // Split form of load_pair for software pipelining.  fetch_pair issues the
+    // global loads of a pair's first-pass inputs (zeros where inactive,
+    // invalid or truncated); finish_pair forms the Hermitian-extended
+    // (X1 + i X2) from them.  kz_regs holds kz of each input slot (the same
+    // for every pair).
+    static constexpr int NIN = FI::B0 * FI::R0;
+    __device__ __forceinline__ static void kz_regs(TB* kz, const SpecTables& tab, int t) {
+#pragma unroll
+        for (int q = 0; q < FI::B0; q++) {
+#pragma unroll
+            for (int r = 0; r < FI::R0; r++) {
+                const int n = FI::in_index(t, q, r);
+                const int sl = n <= NZ / 2 ? n : NZ - n;
+                kz[q * FI::R0 + r] = FI::in_active(t, q) ? (TB)tab.kz[sl] : (TB)0;
+            }
+        }
+    }
+    __device__ __forceinline__ static void fetch_pair(cplx<TB>* r1, cplx<TB>* r2, const cplx<TB>* __restrict__ s1,
+                                                      const cplx<TB>* __restrict__ s2, bool valid,
+                                                      const SpecTables& tab, int t) {
+#pragma unroll
+        for (int q = 0; q < FI::B0; q++) {
+#pragma unroll
+            for (int r = 0; r < FI::R0; r++) {
+                const int n = FI::in_index(t, q, r);
+                const int sl = n <= NZ / 2 ? n : NZ - n;
+                cplx<TB> a = mk<TB>(0, 0), b = mk<TB>(0, 0);
+                if (FI::in_active(t, q) && valid && (!TRUNC || sl <= tab.lmax)) {
+                    a = s1[sl];
+                    b = s2[sl];
+                }
+                r1[q * FI::R0 + r] = a;
+                r2[q * FI::R0 + r] = b;
+            }
+        }
+    }
+    __device__ __forceinline__ static void finish_pair(cplx<TB>* v, const cplx<TB>* r1, bool d1, const cplx<TB>* r2,
+                                                       bool d2, const TB* kz, int t) {
+#pragma unroll
+        for (int q = 0; q < FI::B0; q++) {
+#pragma unroll
+            for (int r = 0; r < FI::R0; r++) {
+                const int i = q * FI::R0 + r;
+                const int n = FI::in_index(t, q, r);
+                cplx<TB> a = r1[i], b = r2[i];
+                if (d1) a = mk<TB>(-a.y * kz[i], a.x * kz[i]);
+                if (d2) b = mk<TB>(-b.y * kz[i], b.x * kz[i]);
+                if (n == 0 || n == NZ / 2) {
+                    a.y = 0;
+                    b.y = 0;
+                } else if (n > NZ / 2) {
+                    a.y = -a.y;
+                    b.y = -b.y;
+                }
+                v[i] = mk<TB>(a.x - b.y, a.y + b.x);
             }
         }
     }
@@ -315,7 +401,50 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
     auto fld = [&](int f) { return Bf + (long)f * nc + off; };
     // field ids: s*9 + {0..2: w_b, 3..5: dx w_b, 6..8: dy w_b}
     TB Q[3][NPT];
-    if constexpr (NF == 1) {
+#ifndef NS3D_ZPREFETCH
+#define NS3D_ZPREFETCH 1
+#endif
+    if constexpr (NF == 1 && NS3D_ZPREFETCH) {
+        // Software-pipelined: the loads of pair i+1 are issued before the
+        // transform of pair i, so HBM latency overlaps the LDS passes.
+        TB w[3][NPT], a[NPT], b[NPT], keep[NPT];
+        TB kz[Z::NIN];
+        cplx<TB> r1[Z::NIN], r2[Z::NIN];
+        Z::kz_regs(kz, tab, t);
+        Z::fetch_pair(r1, r2, fld(0), fld(1), valid, tab, t);
+        auto run_pair = [&](TB* x1, TB* x2, bool d1, bool d2, const cplx<TB>* n1, const cplx<TB>* n2) {
+            cplx<TB> v[Z::E];
+            Z::finish_pair(v, r1, d1, r2, d2, kz, t);
+            if (n1) Z::fetch_pair(r1, r2, n1, n2, valid, tab, t);
+            Z::FI::run(v, lds, tw, t, p);
+#pragma unroll
+            for (int e = 0; e < NPT; e++) {
+                x1[e] = v[e].x * scale;
+                x2[e] = v[e].y * scale;
+            }
+        };
+        run_pair(w[0], w[1], false, false, fld(2), fld(3));   // w_x, w_y
+        run_pair(w[2], keep, false, false, fld(6), fld(0));   // w_z, dx w_x
+        if (uphys && valid) {
+#pragma unroll
+            for (int e = 0; e < NPT; e++) {
+                const long g = pen * NZ + Z::FI::out_index(t, e / Z::FI::RL, e % Z::FI::RL);
+                uphys[g] = w[0][e];
+                uphys[nr + g] = w[1][e];
+                uphys[2 * nr + g] = w[2][e];
+            }
+        }
+        run_pair(a, b, false, true, fld(4), fld(7));          // dy w_x, dz w_x
+#pragma unroll
+        for (int e = 0; e < NPT; e++) Q[0][e] = w[0][e] * keep[e] + w[1][e] * a[e] + w[2][e] * b[e];
+        run_pair(keep, a, false, false, fld(1), fld(5));      // dx w_y, dy w_y
+        run_pair(b, Q[2], true, false, fld(8), fld(2));       // dz w_y, dx w_z
+#pragma unroll
+        for (int e = 0; e < NPT; e++) Q[1][e] = w[0][e] * keep[e] + w[1][e] * a[e] + w[2][e] * b[e];
+        run_pair(a, b, false, true, nullptr, nullptr);        // dy w_z, dz w_z
+#pragma unroll
+        for (int e = 0; e < NPT; e++) Q[2][e] = w[0][e] * Q[2][e] + w[1][e] * a[e] + w[2][e] * b[e];
+    } else if constexpr (NF == 1) {
         TB w[3][NPT], a[NPT], b[NPT], keep[NPT];
         Z::c2r_pair(w[0], w[1], fld(0), false, fld(1), false, valid, tab, lds, tw, t, p, scale);
         Z::c2r_pair(w[2], keep, fld(2), false, fld(3), false, valid, tab, lds, tw, t, p, scale);  // w_z, dx w_x
@@ -340,28 +469,49 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[2][e] = w[0][e] * Q[2][e] + w[1][e] * a[e] + w[2][e] * b[e];
     } else {
-        // Q_b = (v.grad) u_b + (u.grad) v_b  (convolute_2p3_UV, Source/advection_2_3.cu:401-475)
+        // Q_c = (v.grad) u_c + (u.grad) v_c  (convolute_2p3_UV, Source/advection_2_3.cu:401-475).
+        // Each paired transform holds two u-derived or two v-derived fields,
+        // never one of each: v (a Krylov vector) can be orders of magnitude
+        // smaller than u, and a shared transform's rounding is relative to
+        // the larger of its two fields.
+        // ids: u_b = b, dx u_b = 3+b, dy u_b = 6+b; v_b = 9+b, dx v_b = 12+b, dy v_b = 15+b
         TB u[3][NPT], vv[3][NPT], a[NPT], b[NPT];
         Z::c2r_pair(u[0], u[1], fld(0), false, fld(1), false, valid, tab, lds, tw, t, p, scale);
-        Z::c2r_pair(u[2], vv[0], fld(2), false, fld(9), false, valid, tab, lds, tw, t, p, scale);
-        Z::c2r_pair(vv[1], vv[2], fld(10), false, fld(11), false, valid, tab, lds, tw, t, p, scale);
+        Z::c2r_pair(vv[0], vv[1], fld(9), false, fld(10), false, valid, tab, lds, tw, t, p, scale);
+        Z::c2r_pair(u[2], a, fld(2), false, fld(3), false, valid, tab, lds, tw, t, p, scale);      // u_z, dx u_x
+        Z::c2r_pair(vv[2], b, fld(11), false, fld(12), false, valid, tab, lds, tw, t, p, scale);   // v_z, dx v_x
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            // (v.grad) u_c
-            Z::c2r_pair(a, b, fld(3 + c), false, fld(6 + c), false, valid, tab, lds, tw, t, p, scale);
+        for (int e = 0; e < NPT; e++) Q[0][e] = vv[0][e] * a[e] + u[0][e] * b[e];
+        Z::c2r_pair(a, b, fld(6), false, fld(0), true, valid, tab, lds, tw, t, p, scale);          // dy u_x, dz u_x
 #pragma unroll
-            for (int e = 0; e < NPT; e++) Q[c][e] = vv[0][e] * a[e] + vv[1][e] * b[e];
-            Z::c2r_pair(a, b, fld(c), true, fld(9 + 3 + c), false, valid, tab, lds, tw, t, p, scale);  // dz u_c, dx v_c
+        for (int e = 0; e < NPT; e++) Q[0][e] += vv[1][e] * a[e] + vv[2][e] * b[e];
+        Z::c2r_pair(a, b, fld(15), false, fld(9), true, valid, tab, lds, tw, t, p, scale);         // dy v_x, dz v_x
 #pragma unroll
-            for (int e = 0; e < NPT; e++) Q[c][e] = Q[c][e] + vv[2][e] * a[e];
-            TB cdx[NPT];
+        for (int e = 0; e < NPT; e++) Q[0][e] += u[1][e] * a[e] + u[2][e] * b[e];
+        Z::c2r_pair(a, b, fld(4), false, fld(7), false, valid, tab, lds, tw, t, p, scale);         // dx u_y, dy u_y
 #pragma unroll
-            for (int e = 0; e < NPT; e++) cdx[e] = b[e];
-            // (u.grad) v_c
-            Z::c2r_pair(a, b, fld(9 + 6 + c), false, fld(9 + c), true, valid, tab, lds, tw, t, p, scale);  // dy v_c, dz v_c
+        for (int e = 0; e < NPT; e++) Q[1][e] = vv[0][e] * a[e] + vv[1][e] * b[e];
+        Z::c2r_pair(a, b, fld(1), true, fld(5), false, valid, tab, lds, tw, t, p, scale);          // dz u_y, dx u_z
 #pragma unroll
-            for (int e = 0; e < NPT; e++) Q[c][e] = Q[c][e] + (u[0][e] * cdx[e] + u[1][e] * a[e] + u[2][e] * b[e]);
+        for (int e = 0; e < NPT; e++) {
+            Q[1][e] += vv[2][e] * a[e];
+            Q[2][e] = vv[0][e] * b[e];
         }
+        Z::c2r_pair(a, b, fld(13), false, fld(16), false, valid, tab, lds, tw, t, p, scale);       // dx v_y, dy v_y
+#pragma unroll
+        for (int e = 0; e < NPT; e++) Q[1][e] += u[0][e] * a[e] + u[1][e] * b[e];
+        Z::c2r_pair(a, b, fld(10), true, fld(14), false, valid, tab, lds, tw, t, p, scale);        // dz v_y, dx v_z
+#pragma unroll
+        for (int e = 0; e < NPT; e++) {
+            Q[1][e] += u[2][e] * a[e];
+            Q[2][e] += u[0][e] * b[e];
+        }
+        Z::c2r_pair(a, b, fld(8), false, fld(2), true, valid, tab, lds, tw, t, p, scale);          // dy u_z, dz u_z
+#pragma unroll
+        for (int e = 0; e < NPT; e++) Q[2][e] += vv[1][e] * a[e] + vv[2][e] * b[e];
+        Z::c2r_pair(a, b, fld(17), false, fld(11), true, valid, tab, lds, tw, t, p, scale);        // dy v_z, dz v_z
+#pragma unroll
+        for (int e = 0; e < NPT; e++) Q[2][e] += u[1][e] * a[e] + u[2][e] * b[e];
     }
     cplx<TB>* o0 = C + off;
     cplx<TB>* o1 = C + nc + off;
@@ -418,13 +568,15 @@ k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, i
 }
 
 // ---------------------------------------------------------------- P5 epilogues
-enum EpiMode { EPI_Q = 0, EPI_STAGE = 1, EPI_RHS = 2, EPI_UV = 3, EPI_UV_RHS = 4, EPI_IUV = 5 };
+// EPI_STAGE32: EPI_STAGE with w, u0 and out in FP32 (the FP32-state RK3 loop);
+// the arithmetic and the forcing stay FP64.
+enum EpiMode { EPI_Q = 0, EPI_STAGE = 1, EPI_RHS = 2, EPI_UV = 3, EPI_UV_RHS = 4, EPI_IUV = 5, EPI_STAGE32 = 6 };
 
 struct EpiArgs {
-    cplx<double>* w[3];          // state (read; written in place for UV/UV_RHS/IUV)
-    const cplx<double>* u0[3];   // SSP combination base (EPI_STAGE, may be null)
+    cplx<double>* w[3];          // state (read; written in place for UV/UV_RHS/IUV); cplx<float> for EPI_STAGE32
+    const cplx<double>* u0[3];   // SSP combination base (EPI_STAGE, may be null); cplx<float> for EPI_STAGE32
     const cplx<double>* f[3];    // forcing (EPI_STAGE / EPI_RHS, may be null)
-    cplx<double>* out[3];        // output (EPI_Q, EPI_STAGE, EPI_RHS)
+    cplx<double>* out[3];        // output (EPI_Q, EPI_STAGE, EPI_RHS); cplx<float> for EPI_STAGE32
     double dt, Re, a0, a1;       // EPI_STAGE: out = a0*u0 + a1*E(w)
     int ipq;                     // reference IPQ rule (Q1)
 };
@@ -448,8 +600,23 @@ __device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& ta
     const double dt = ea.dt, Re = ea.Re;
 #pragma unroll
     for (int b = 0; b < 3; b++) {
-        const cplx<double> w = ea.w[b][g];
         cplx<double> r;
+        if (MODE == EPI_STAGE32) {
+            const cplx<double> w = cvt<double>(reinterpret_cast<const cplx<float>*>(ea.w[b])[g]);
+            const double G = (1.0 - c.din * dt / Re);
+            cplx<double> f = mk<double>(0.0, 0.0);
+            if (ea.f[b]) f = ea.f[b][g];
+            r.x = dt * AQr[b] + G * w.x + dt * f.x - dt * Q[b].x;
+            r.y = dt * AQi[b] + G * w.y + dt * f.y - dt * Q[b].y;
+            if (ea.u0[b]) {
+                const cplx<double> u0 = cvt<double>(reinterpret_cast<const cplx<float>*>(ea.u0[b])[g]);
+                r.x = ea.a0 * u0.x + ea.a1 * r.x;
+                r.y = ea.a0 * u0.y + ea.a1 * r.y;
+            }
+            reinterpret_cast<cplx<float>*>(ea.out[b])[g] = cvt<float>(r);
+            continue;
+        }
+        const cplx<double> w = ea.w[b][g];
         if (MODE == EPI_STAGE) {
             const double G = (1.0 - c.din * dt / Re);
             cplx<double> f = mk<double>(0.0, 0.0);
@@ -520,6 +687,7 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int nlt, in
     const long S = (long)Ny * Mz;
     const long base = (long)k * Mz + l;
     cplx<TB> qv[3][F::E];
+    // the three components are all live until the epilogue: load them together
 #pragma unroll
     for (int b = 0; b < 3; b++) {
         const cplx<TB>* src = D + (long)b * nc;
@@ -532,7 +700,11 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int nlt, in
                 qv[b][q * F::R0 + r] = s;
             }
         }
-        F::run(qv[b], lds, tw, t, p);
+        if (!NS3D_HOIST) F::run(qv[b], lds, tw, t, p);
+    }
+    if (NS3D_HOIST) {
+#pragma unroll
+        for (int b = 0; b < 3; b++) F::run(qv[b], lds, tw, t, p);
     }
     if (!lv) return;
 #pragma unroll

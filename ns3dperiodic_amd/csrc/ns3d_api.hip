@@ -89,6 +89,7 @@ struct ns3d_ctx {
     // FP64 stage buffers and a Q^ buffer
     zd* stage = nullptr;  // 9 fields: u1[3], u2[3], q[3]
     zd* extra = nullptr;  // 6 fields: tangent map v1[3] + spare
+    cplx<float>* s32 = nullptr;  // NS3D_FP32_STATE RK3 loop: state[3], u1[3], u2[3] in FP32
     double* red = nullptr;  // reduction partials (2*RED_BLOCKS + 2)
     double* red_host = nullptr;  // pinned
     void* real_scratch = nullptr;  // physical-space scratch (FP64 or FP32), 4 fields
@@ -172,7 +173,10 @@ int build_tables(ns3d_ctx* c) {
     (void)mz;
     std::vector<double> mxv(Nx), myv(Ny), mzv(Mz);
     const double k1 = 2.0 * M_PI / Lx, k2 = 2.0 * M_PI / Ly, k3 = 2.0 * M_PI / Lz;
-    const double kxMax = 2 * Nx * PI / Lx, kyMax = 2 * Ny * PI / Ly, kzMax = 2 * Mz * PI / Lz;
+    // build_mask_matrix is called with (Nx, Ny, Mz); NS3D_FLAG_MASK_NYQUIST uses the Nyquist wavenumbers
+    const bool nyq = (c->flags & NS3D_FLAG_MASK_NYQUIST) != 0;
+    const double kxMax = nyq ? Nx * PI / Lx : 2 * Nx * PI / Lx, kyMax = nyq ? Ny * PI / Ly : 2 * Ny * PI / Ly,
+                 kzMax = nyq ? c->Nz * PI / Lz : 2 * Mz * PI / Lz;
     for (int j = 0; j < Nx; j++) {
         const int m = j >= Nx / 2 ? j - Nx : j;
         kx[j] = 2.0 * M_PI / Lx * m;            // build_Nabla_Wavenumbers (math_support.cu:393-414)
@@ -297,8 +301,8 @@ inline dim3 xgrid(long nb, int nfy = 1) { return dim3((unsigned)(((nb + 7) / 8) 
 
 // ---------------------------------------------------------------- fused pipeline
 // Runs P1..P4 for NF vector fields and leaves D (3 fields) at scratch + 3*nc.
-template <typename TB>
-int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
+template <typename TB, typename TS = double>
+int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf) {
     int st = ensure_scratch(c, 15 * nf);
     if (st) return st;
     cplx<TB>* A = (cplx<TB>*)c->scratch;
@@ -316,11 +320,14 @@ int run_nonlinear_passes(ns3d_ctx* c, const zd* const* in, int nf) {
         constexpr int OS = NS3D_OS_XINV;
         dim3 grid = xgrid(nb, 3 * nf * (OS ? 2 : 1)), blk(P * XYPlan<NX, E>::TPP);
         if (nf == 1) {
-            PtrArr<const zd, 3> w{{in[0], in[1], in[2]}};
-            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E, OS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 0));
-        } else {
+            PtrArr<const cplx<TS>, 3> w{{in[0], in[1], in[2]}};
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E, OS, TS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab,
+                               twp<TB>(c, 0));
+        } else if constexpr (std::is_same<TS, double>::value) {
             PtrArr<const zd, 6> w{{in[0], in[1], in[2], in[3], in[4], in[5]}};
             hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2, E, OS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 0));
+        } else {
+            return NS3D_ERR_ARG;
         }
         return NS3D_OK;
     });
@@ -454,6 +461,76 @@ int rk3_one(ns3d_ctx* c, zd* const u[3], const zd* const f[3]) {
     if ((st = stage_into(c, uc, nullptr, 0.0, 1.0, u1, f))) return st;
     if ((st = stage_into(c, u1c, uc, 3.0 / 4.0, 1.0 / 4.0, u2, f))) return st;
     if ((st = stage_into(c, u2c, uc, 1.0 / 3.0, 2.0 / 3.0, u, f))) return st;
+    return NS3D_OK;
+}
+
+// ---- NS3D_FP32_STATE: the RK3 loop keeps state and stages in FP32 (bulk
+// transforms FP32, epilogue arithmetic and forcing FP64); the caller's FP64
+// state is converted once on entry and once on exit of each call.
+using cf = cplx<float>;
+
+int stage_into32(ns3d_ctx* c, const cf* const w[3], const cf* const u0[3], double a0, double a1, cf* const out[3],
+                 const zd* const f[3]) {
+    EpiArgs ea = make_epi(c);
+    for (int b = 0; b < 3; b++) {
+        ea.w[b] = (zd*)w[b];
+        ea.u0[b] = u0 ? (const zd*)u0[b] : nullptr;
+        ea.f[b] = f ? f[b] : nullptr;
+        ea.out[b] = (zd*)out[b];
+    }
+    ea.a0 = a0;
+    ea.a1 = a1;
+    int st = run_nonlinear_passes<float, float>(c, w, 1);
+    if (st) return st;
+    st = run_x_forward_epilogue<float, EPI_STAGE32>(c, ea);
+    if (st) return st;
+    return hipGetLastError() == hipSuccess ? NS3D_OK : NS3D_ERR_HIP;
+}
+
+int rk3_one32(ns3d_ctx* c, const zd* const f[3]) {
+    const long nc = c->nc;
+    cf* u[3] = {c->s32, c->s32 + nc, c->s32 + 2 * nc};
+    cf* u1[3] = {c->s32 + 3 * nc, c->s32 + 4 * nc, c->s32 + 5 * nc};
+    cf* u2[3] = {c->s32 + 6 * nc, c->s32 + 7 * nc, c->s32 + 8 * nc};
+    const cf* uc[3] = {u[0], u[1], u[2]};
+    const cf* u1c[3] = {u1[0], u1[1], u1[2]};
+    const cf* u2c[3] = {u2[0], u2[1], u2[2]};
+    int st;
+    if ((st = stage_into32(c, uc, nullptr, 0.0, 1.0, u1, f))) return st;
+    if ((st = stage_into32(c, u1c, uc, 3.0 / 4.0, 1.0 / 4.0, u2, f))) return st;
+    if ((st = stage_into32(c, u2c, uc, 1.0 / 3.0, 2.0 / 3.0, u, f))) return st;
+    return NS3D_OK;
+}
+
+int ensure_state32(ns3d_ctx* c) {
+    if (c->s32) return NS3D_OK;
+    if (hipMalloc((void**)&c->s32, sizeof(cf) * (size_t)c->nc * 9) != hipSuccess) return NS3D_ERR_ALLOC;
+    return NS3D_OK;
+}
+
+// nsteps RK3 steps in the context's precision
+int rk3_steps(ns3d_ctx* c, int nsteps, zd* const u[3], const zd* const f[3]) {
+    if (c->precision != NS3D_FP32_STATE) {
+        for (int i = 0; i < nsteps; i++) {
+            int st = rk3_one(c, u, f);
+            if (st) return st;
+        }
+        return NS3D_OK;
+    }
+    if (nsteps == 0) return NS3D_OK;
+    int st = ensure_state32(c);
+    if (!st) st = ensure_scratch(c, 15);
+    if (st) return st;
+    const long nc = c->nc;
+    for (int b = 0; b < 3; b++)
+        hipLaunchKernelGGL(k_d2f_c, dim3(nblk(nc, 256)), dim3(256), 0, c->stream, u[b], c->s32 + b * nc, nc);
+    for (int i = 0; i < nsteps; i++) {
+        st = rk3_one32(c, f);
+        if (st) return st;
+    }
+    for (int b = 0; b < 3; b++)
+        hipLaunchKernelGGL(k_f2d_c, dim3(nblk(nc, 256)), dim3(256), 0, c->stream, c->s32 + b * nc, u[b], nc);
+    HIPCHK(hipGetLastError());
     return NS3D_OK;
 }
 
@@ -659,7 +736,8 @@ int ns3d_create(const ns3d_config* cfg, ns3d_ctx** out) {
     *out = nullptr;
     if (!size_ok(cfg->Nx) || !size_ok(cfg->Ny) || !size_ok(cfg->Nz)) return NS3D_ERR_SIZE;
     if (!(cfg->Lx > 0) || !(cfg->Ly > 0) || !(cfg->Lz > 0) || !(cfg->Re > 0)) return NS3D_ERR_ARG;
-    if (cfg->precision != NS3D_FP64 && cfg->precision != NS3D_FP32_BULK) return NS3D_ERR_ARG;
+    if (cfg->precision != NS3D_FP64 && cfg->precision != NS3D_FP32_BULK && cfg->precision != NS3D_FP32_STATE)
+        return NS3D_ERR_ARG;
     if (cfg->device >= 0) HIPCHK(hipSetDevice(cfg->device));
     ns3d_ctx* c = new (std::nothrow) ns3d_ctx();
     if (!c) return NS3D_ERR_ALLOC;
@@ -702,6 +780,7 @@ int ns3d_destroy(ns3d_ctx* c) {
     }
     hipFree(c->scratch);
     hipFree(c->stage);
+    hipFree(c->s32);
     hipFree(c->extra);
     hipFree(c->red);
     hipFree(c->real_scratch);
@@ -743,6 +822,7 @@ long long ns3d_workspace_bytes(const ns3d_ctx* c) {
     if (c->stage) b += (long long)sizeof(zd) * c->nc * 9;
     if (c->extra) b += (long long)sizeof(zd) * c->nc * 6;
     if (c->kry) b += (long long)sizeof(zd) * c->nc * 3;
+    if (c->s32) b += (long long)sizeof(cplx<float>) * c->nc * 9;
     if (c->real_scratch) b += (long long)sizeof(double) * c->nr * 4;
     return b;
 }
@@ -921,11 +1001,7 @@ int ns3d_rk3_ssp_steps(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns
     if (!c || !ux || !uy || !uz || nsteps < 0) return NS3D_ERR_ARG;
     zd* u[3] = {(zd*)ux, (zd*)uy, (zd*)uz};
     const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
-    for (int i = 0; i < nsteps; i++) {
-        int st = rk3_one(c, u, f);
-        if (st) return st;
-    }
-    return NS3D_OK;
+    return rk3_steps(c, nsteps, u, f);
 }
 
 int ns3d_rk3_ssp_scalar(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns3d_cplx* uz, const ns3d_cplx* fx,
@@ -951,7 +1027,7 @@ int ns3d_rk3_ssp_graph(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns
         drop_graph(c);
         // every buffer the step uses is allocated before capture
         int st = ensure_scratch(c, 15);
-        if (!st) st = ensure_fp64(c, &c->stage, 9);
+        if (!st) st = c->precision == NS3D_FP32_STATE ? ensure_state32(c) : ensure_fp64(c, &c->stage, 9);
         if (st) return st;
         HIPCHK(hipStreamSynchronize(c->stream));
         const bool prof = c->prof_on;
@@ -959,7 +1035,7 @@ int ns3d_rk3_ssp_graph(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         zd* u[3] = {(zd*)ux, (zd*)uy, (zd*)uz};
         const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
-        for (int i = 0; i < nsteps && !st; i++) st = rk3_one(c, u, f);
+        st = rk3_steps(c, nsteps, u, f);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(c->stream, &g);
         c->prof_on = prof;

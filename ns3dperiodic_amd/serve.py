@@ -16,6 +16,15 @@ overlaps the kernels of all active requests.  A request's arithmetic is the
 same kernels in the same order whether it runs alone or in a mix, so its
 result is bitwise the same either way.
 
+Requests run in the physical mode (``ipq=False``: NS3D_FLAG_NO_IPQ and
+NS3D_FLAG_MASK_NYQUIST).  Zeroing Re(Q^) (Source/advection_2_3.cu:35-70) is
+exact only for fields with purely imaginary coefficients (odd symmetry, the
+reference's Kolmogorov states), and the reference's mask, normalised by
+(Nx, Ny, Mz) (Source/math_support.cu:471-519), leaves the x and y axes
+untruncated; on the stream's generic random fields at Re up to 500 on grids
+down to 32^3 either one lets aliasing or the IPQ rule feed energy in and
+solves diverge.  With both off the discrete nonlinear term conserves energy.
+
 Scheduling (``Ledger``, ``poisson_stream``, ``choose_dt``) is plain Python
 without GPU calls; ``Server`` and ``one_at_a_time`` drive the HIP library.
 """
@@ -54,12 +63,15 @@ class Result:
     v: list = None     # final perturbation (jv)
 
 
-def choose_dt(u_hat, T, N, cfl=0.2, L=2 * math.pi):
-    """dt = cfl * min dx / max|u0|, shrunk so that T/dt is an integer.
-    Returns (nsteps, dt).  max|u0| is the largest velocity component in
+def choose_dt(u_hat, T, N, cfl=0.2, L=2 * math.pi, fmax=0.0):
+    """dt = cfl * dx / U, shrunk so that T/dt is an integer; returns (nsteps, dt).
+    U = max|u0| + T*fmax bounds the velocity over the horizon (forcing of
+    amplitude fmax can raise it by at most T*fmax), so the CFL number stays
+    <= cfl for the whole solve.  max|u0| is the largest velocity component in
     physical space (as calc_dt uses, Source/cuda_supp.cu:170-238)."""
     umax = max(float(np.max(np.abs(np.fft.irfftn(c, s=(N, N, N), axes=(0, 1, 2))))) for c in u_hat)
-    dt0 = cfl * (L / N) / max(umax, 1e-30)
+    U = umax + T * fmax
+    dt0 = cfl * (L / N) / max(U, 1e-30)
     n = max(1, int(math.ceil(T / dt0 - 1e-12)))
     return n, T / n
 
@@ -94,7 +106,7 @@ def request_inputs(req, energy=0.5):
         v = fields.random_solenoidal(N, seed=req.seed + 7919, energy=1.0)
         nrm = math.sqrt(sum(float(np.vdot(c, c).real) for c in v))
         v = [c / nrm for c in v]
-    req.nsteps, req.dt = choose_dt(u, req.T, N)
+    req.nsteps, req.dt = choose_dt(u, req.T, N, fmax=1.0)
     return u, fx, v
 
 
@@ -183,13 +195,13 @@ class _Slot:
     """A context, a stream and resident buffers for one request at a time;
     reused by later requests of the same size and kind."""
 
-    def __init__(self, N, kind, precision, device, chunk, use_graphs):
+    def __init__(self, N, kind, precision, device, chunk, use_graphs, ipq):
         import torch
         from .solver import NS3D
         self.N, self.kind = N, kind
         self.bytes = request_bytes(Request(-1, kind, N, 1.0, 1.0, 0, 0.0), precision)
         self.stream = torch.cuda.Stream(device=device)
-        self.g = NS3D(N, precision=precision, stream=self.stream, device=device)
+        self.g = NS3D(N, precision=precision, stream=self.stream, device=device, ipq=ipq, mask_nyquist=not ipq)
         cs = (N, N, N // 2 + 1)
         dev = torch.device("cuda", device)
         self.u = [torch.empty(cs, dtype=torch.complex128, device=dev) for _ in range(3)]
@@ -250,8 +262,9 @@ class Server:
     on its own slot (context + stream), admitted by HBM budget."""
 
     def __init__(self, precision="fp32", max_active=8, budget_bytes=64 << 30, chunk=16, use_graphs=True,
-                 device=0, inflight=2):
+                 device=0, inflight=2, ipq=False):
         self.precision = precision
+        self.ipq = ipq
         self.ledger = Ledger(budget_bytes, max_active)
         self.chunk = chunk
         self.use_graphs = use_graphs
@@ -266,7 +279,7 @@ class Server:
             s = self.idle[key].pop()
             self.idle_bytes -= s.bytes
             return s
-        return _Slot(r.N, r.kind, self.precision, self.device, self.chunk, self.use_graphs)
+        return _Slot(r.N, r.kind, self.precision, self.device, self.chunk, self.use_graphs, self.ipq)
 
     def _make_room(self, need):
         # drop idle slots (largest first) while the budget is short
@@ -347,7 +360,7 @@ def pin_io(u, fx, v):
     return io
 
 
-def one_at_a_time(requests, io, precision="fp64", device=0, clock=time.perf_counter, ctxs=None):
+def one_at_a_time(requests, io, precision="fp64", device=0, clock=time.perf_counter, ctxs=None, ipq=False):
     """Baseline: requests in arrival order, one at a time, eager RK3 steps on
     one context per grid size (contexts reused; pass `ctxs` to keep them
     across calls), upload and download included."""
@@ -363,7 +376,7 @@ def one_at_a_time(requests, io, precision="fp64", device=0, clock=time.perf_coun
             time.sleep(50e-6)
         adm = clock() - t0
         if r.N not in ctxs:
-            ctxs[r.N] = NS3D(r.N, precision=precision, device=device)
+            ctxs[r.N] = NS3D(r.N, precision=precision, device=device, ipq=ipq, mask_nyquist=not ipq)
         g = ctxs[r.N]
         g.set_params(r.dt, r.Re)
         inp = io[r.rid]

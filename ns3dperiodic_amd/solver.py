@@ -32,7 +32,8 @@ class NS3D:
     """One grid/box configuration; owns an ns3d_ctx (struct_NS3D_RK3_Call analogue)."""
 
     def __init__(self, Nx, Ny=None, Nz=None, Lx=2 * math.pi, Ly=2 * math.pi, Lz=2 * math.pi,
-                 Re=100.0, dt=1e-3, precision="fp64", ipq=True, weno_zfix=False, device=0, stream=None):
+                 Re=100.0, dt=1e-3, precision="fp64", ipq=True, weno_zfix=False, device=0, stream=None,
+                 mask_nyquist=False):
         self.lib = _lib.load()
         Ny = Nx if Ny is None else Ny
         Nz = Nx if Nz is None else Nz
@@ -40,8 +41,10 @@ class NS3D:
         self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
         self.Re, self.dt = Re, dt
         self.device = torch.device("cuda", device)
-        flags = (0 if ipq else _lib.NS3D_FLAG_NO_IPQ) | (_lib.NS3D_FLAG_WENO_ZFIX if weno_zfix else 0)
-        prec = {"fp64": _lib.NS3D_FP64, "fp32": _lib.NS3D_FP32_BULK}[precision]
+        flags = ((0 if ipq else _lib.NS3D_FLAG_NO_IPQ) | (_lib.NS3D_FLAG_WENO_ZFIX if weno_zfix else 0)
+                 | (_lib.NS3D_FLAG_MASK_NYQUIST if mask_nyquist else 0))
+        # "fp32": FP32 transforms, FP64 state; "fp32s": also FP32 state inside the RK3 loops
+        prec = {"fp64": _lib.NS3D_FP64, "fp32": _lib.NS3D_FP32_BULK, "fp32s": _lib.NS3D_FP32_STATE}[precision]
         self.precision = precision
         if stream is None:
             stream = torch.cuda.current_stream(self.device)
@@ -96,7 +99,16 @@ class NS3D:
         return [_ptr(c) for c in u]
 
     def _f3(self, f):
-        return [None, None, None] if f is None else self._c3(f)
+        """Forcing: None, or three components of which any may be None (zero)."""
+        if f is None:
+            return [None, None, None]
+        assert len(f) == 3
+        out = []
+        for c in f:
+            if c is not None:
+                assert c.dtype == torch.complex128 and tuple(c.shape) == self.cshape, (c.dtype, c.shape)
+            out.append(_ptr(c))
+        return out
 
     def set_params(self, dt=None, Re=None):
         self.dt = self.dt if dt is None else dt

@@ -267,7 +267,9 @@ void or_tables(const or_params *p, double *kx, double *ky, double *kz,
                 /* build_mask_matrix (:471-519), called with Nz := Mz (Q2). */
                 if (mask) {
                     const double kxm = m * 2 * OR_PI / Lx, kym = n * 2 * OR_PI / Ly, kzm = q * 2 * OR_PI / Lz;
-                    const double kxMax = 2 * Nx * OR_PI / Lx, kyMax = 2 * Ny * OR_PI / Ly, kzMax = 2 * Mz * OR_PI / Lz;
+                    const double kxMax = p->mask_nyq ? Nx * OR_PI / Lx : 2 * Nx * OR_PI / Lx;
+                    const double kyMax = p->mask_nyq ? Ny * OR_PI / Ly : 2 * Ny * OR_PI / Ly;
+                    const double kzMax = p->mask_nyq ? p->Nz * OR_PI / Lz : 2 * Mz * OR_PI / Lz;
                     const double s2 = sqrt(kxm * kxm / (kxMax * kxMax) + kym * kym / (kyMax * kyMax) + kzm * kzm / (kzMax * kzMax));
                     mask[id] = s2 < 2.0 / 3.0 ? 1.0 : 0.0;
                 }

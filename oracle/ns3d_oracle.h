@@ -28,6 +28,7 @@ typedef struct {
     int ipq;               /* 1: zero Re(Q^) and Q^(0) after every nonlinear term (reference, advection_2_3.cu:35-70) */
     int weno_zfix;         /* 0: keep the l-1 stencil typo of flux_z_weno_R (advection_WENO.cu:288); 1: use l-2 */
     int nthreads;          /* OpenMP threads (<=0: library default) */
+    int mask_nyq;          /* 1: 2/3 mask relative to each axis's Nyquist wavenumber (not the reference) */
 } or_params;
 
 /* ---- FFT layer (math_support.cu:301-334) ---- */

@@ -24,7 +24,8 @@ class Params(ctypes.Structure):
     _fields_ = [("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("Nz", ctypes.c_int),
                 ("Lx", ctypes.c_double), ("Ly", ctypes.c_double), ("Lz", ctypes.c_double),
                 ("Re", ctypes.c_double), ("dt", ctypes.c_double),
-                ("ipq", ctypes.c_int), ("weno_zfix", ctypes.c_int), ("nthreads", ctypes.c_int)]
+                ("ipq", ctypes.c_int), ("weno_zfix", ctypes.c_int), ("nthreads", ctypes.c_int),
+                ("mask_nyq", ctypes.c_int)]
 
 
 def build(force=False):
@@ -60,18 +61,18 @@ class Oracle:
     """One grid/box configuration of the CPU oracle."""
 
     def __init__(self, Nx, Ny=None, Nz=None, Lx=2 * np.pi, Ly=2 * np.pi, Lz=2 * np.pi,
-                 Re=100.0, dt=1e-3, ipq=True, weno_zfix=False, nthreads=0):
+                 Re=100.0, dt=1e-3, ipq=True, weno_zfix=False, nthreads=0, mask_nyq=False):
         Ny = Nx if Ny is None else Ny
         Nz = Nx if Nz is None else Nz
         self.Nx, self.Ny, self.Nz, self.Mz = Nx, Ny, Nz, Nz // 2 + 1
         self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
         self.Re, self.dt = Re, dt
-        self.ipq, self.weno_zfix, self.nthreads = ipq, weno_zfix, nthreads
+        self.ipq, self.weno_zfix, self.nthreads, self.mask_nyq = ipq, weno_zfix, nthreads, mask_nyq
 
     @property
     def params(self):
         return Params(self.Nx, self.Ny, self.Nz, self.Lx, self.Ly, self.Lz, self.Re, self.dt,
-                      int(self.ipq), int(self.weno_zfix), int(self.nthreads))
+                      int(self.ipq), int(self.weno_zfix), int(self.nthreads), int(self.mask_nyq))
 
     @property
     def cshape(self):

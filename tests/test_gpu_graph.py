@@ -12,7 +12,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shape,precision", [((32, 32, 32), "fp64"), ((48, 32, 64), "fp64"), ((64, 64, 64), "fp32")])
+@pytest.mark.parametrize("shape,precision", [((32, 32, 32), "fp64"), ((48, 32, 64), "fp64"), ((64, 64, 64), "fp32"),
+                                             ((32, 48, 32), "fp32s")])
 def test_graph_replays_match_eager(shape, precision):
     import torch
     from ns3dperiodic_amd import NS3D, fields

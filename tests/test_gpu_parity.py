@@ -82,6 +82,25 @@ def test_bilinear(pair):
     assert rel_l2(Qg, Qo) < TOL
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
+def test_bilinear_small_perturbation(precision, tol):
+    """B(u, s v) = s B(u, v) for a perturbation 1e-6 times smaller than u (the
+    Krylov regime): the z pass must not share a transform between u- and
+    v-derived fields, or v inherits u's absolute rounding."""
+    from ns3dperiodic_amd import NS3D
+    from oracle.oracle import Oracle
+    from tests.gpu_util import to_dev, to_host
+    sh = (32, 32, 32)
+    print(f"\n[case] bilinear small perturbation {precision}", flush=True)
+    g = NS3D(*sh, Re=100.0, dt=2e-3, precision=precision)
+    o = Oracle(*sh, Re=100.0, dt=2e-3)
+    u = _field(sh, seed=4)
+    v = _field(sh, seed=5, energy=1e-12)
+    Qg = to_host(g.calculate_convolution_2p3_UV(to_dev(u), to_dev(v)))
+    Qo = o.bilinear(u, v)
+    assert rel_l2(Qg, Qo) < tol
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_project_update(pair, mode):
     from tests.gpu_util import to_dev, to_host
@@ -250,3 +269,37 @@ def test_rk3_with_passive_scalar(shape):
     uo, tho = o.rk3_scalar(u, f, th, scheme=5, steps=2)
     assert rel_l2(to_host(ud), uo) < TOL
     assert rel_l2(to_host([thd])[0], tho) < TOL
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-6), ("fp32s", 2e-6)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 64, 32)], ids=lambda s: "x".join(map(str, s)))
+def test_rk3_fp32_paths(shape, precision, tol):
+    """FP32 paths: five forced RK3 steps in one call against the FP64 oracle
+    (the FP32-state path converts the state once on entry and once on exit)."""
+    from tests.gpu_util import to_dev, to_host
+    print(f"\n[case] rk3 {precision} grid {shape}", flush=True)
+    g, o = _make(shape, precision=precision)
+    u = _field(shape, seed=31)
+    f = fields.kolmogorov_forcing_sin(*shape, n=2)
+    ud = to_dev(u)
+    g.RK3_SSP(ud, to_dev(f), steps=5)
+    assert rel_l2(to_host(ud), o.rk3(u, f, steps=5)) < tol
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 64, 32)], ids=lambda s: "x".join(map(str, s)))
+def test_physical_mode(shape):
+    """NS3D_FLAG_MASK_NYQUIST with the IPQ rule off (the request server's mode):
+    mask, nonlinear term and RK3 steps against the oracle in the same mode."""
+    from ns3dperiodic_amd import NS3D
+    from oracle.oracle import Oracle
+    from tests.gpu_util import to_dev, to_host
+    print(f"\n[case] physical mode grid {shape}", flush=True)
+    g = NS3D(*shape, Re=300.0, dt=2e-3, ipq=False, mask_nyquist=True)
+    o = Oracle(*shape, Re=300.0, dt=2e-3, ipq=False, mask_nyq=True)
+    np.testing.assert_array_equal(to_host([g.tables()["mask"]])[0], o.tables()["mask"])
+    u = _field(shape, seed=41)
+    f = fields.kolmogorov_forcing_sin(*shape, n=4)
+    assert rel_l2(to_host(g.calculate_convolution_2p3(to_dev(u))), o.nonlinear(u)) < TOL
+    ud = to_dev(u)
+    g.RK3_SSP(ud, to_dev(f), steps=3)
+    assert rel_l2(to_host(ud), o.rk3(u, f, steps=3)) < 3 * TOL

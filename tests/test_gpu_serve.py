@@ -80,7 +80,7 @@ def test_fp32_served_vs_fp64_and_oracle():
             assert _rel(r32[r.rid][1], r64[r.rid][1]) < 1e-5
     # one integrate request against the oracle (FP64, forcing f_x = sin 4y)
     r = next(q for q in reqs if q.kind == "integrate" and q.N == min(p.N for p in reqs if p.kind == "integrate"))
-    o = Oracle(r.N, Re=r.Re, dt=r.dt)
+    o = Oracle(r.N, Re=r.Re, dt=r.dt, ipq=False, mask_nyq=True)   # the server's physical mode
     u0 = [c.numpy() for c in io[r.rid]["u"]]
     f = [io[r.rid]["f"].numpy(), np.zeros_like(u0[0]), np.zeros_like(u0[0])]
     ref = o.rk3(u0, f, steps=r.nsteps)
