@@ -45,8 +45,8 @@ template <int N, int E> using XYPlan = typename XYSel<N, E>::type;
 // TS: scalar type of the state w (double, or float for the FP32-state RK3 loop).
 template <typename TB, int NX, int P, int NF, int E, int OS, typename TS = double>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
-k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int nlt, int nb,
-             SpecTables tab, const cplx<TB>* __restrict__ twg) {
+k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int Ms, int nlt,
+             int nb, SpecTables tab, const cplx<TB>* __restrict__ twg) {
     constexpr int TPP = XYPlan<NX, E>::TPP;
     using F = PencilFFT<TB, NX, TPP, +1, typename XYPlan<NX, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NX * P];
@@ -61,8 +61,10 @@ k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc
     const bool keep = lv && column_kept(tab, k, l);
     if (!__syncthreads_or(keep)) return;
     stage_twiddles<TB, NX>(tw, twg);
-    const long S = (long)Ny * Mz;
+    const long S = (long)Ny * Mz;        // state: reference layout
     const long base = (long)k * Mz + l;
+    const long So = (long)Ny * Ms;       // scratch: rows padded to Ms
+    const long baseo = (long)k * Ms + l;
     const int f = OS ? (int)blockIdx.y >> 1 : (int)blockIdx.y;
     const int which = OS ? (int)blockIdx.y & 1 : 0;  // OS: 0 -> w_b, 1 -> dx w_b
     const cplx<TS>* __restrict__ src = w.p[f];
@@ -98,7 +100,7 @@ k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc
             if (!F::out_active(t, q)) continue;
 #pragma unroll
             for (int r = 0; r < F::RL; r++) {
-                const long g = base + F::out_index(t, q, r) * S;
+                const long g = baseo + F::out_index(t, q, r) * So;
                 ow[g] = v[q * F::RL + r];
                 if constexpr (!OS) od[g] = d[q * F::RL + r];
             }
@@ -112,8 +114,8 @@ k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc
 // of the three outputs (A_b is then read by two blocks).
 template <typename TB, int NY, int P, int NF, int E, int OS>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NY, E>::TPP))
-k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, int Ny, int Mz, int nlt, int nb,
-             SpecTables tab, const cplx<TB>* __restrict__ twg) {
+k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, int Ny, int Mz, int Ms, int nlt,
+             int nb, SpecTables tab, const cplx<TB>* __restrict__ twg) {
     constexpr int TPP = XYPlan<NY, E>::TPP;
     using F = PencilFFT<TB, NY, TPP, +1, typename XYPlan<NY, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NY * P];
@@ -126,8 +128,8 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
     const int l = lt * P + p;
     const bool lv = l < Mz && l <= tab.lmax;
     stage_twiddles<TB, NY>(tw, twg);
-    const long S = Mz;
-    const long base = (long)j * Ny * Mz + l;
+    const long S = Ms;
+    const long base = (long)j * Ny * Ms + l;
     const int f = OS ? (int)blockIdx.y / 3 : (int)blockIdx.y;
     const int which = OS ? (int)blockIdx.y % 3 : -1;  // OS: 0 -> w_b, 1 -> dy w_b, 2 -> dx w_b
     const int s_ = f / 3, b = f % 3;
@@ -525,7 +527,7 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
 // place or in place (each block reads its pencils completely before writing).
 template <typename TB, int NA, int P, int DIR, int AXIS, int E>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NA, E>::TPP))
-k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, int nlt, int nb,
+k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, int Ms, int nlt, int nb,
            const cplx<TB>* __restrict__ twg) {
     constexpr int TPP = XYPlan<NA, E>::TPP;
     using F = PencilFFT<TB, NA, TPP, DIR, typename XYPlan<NA, E>::R, IndexMajor<P>>;
@@ -538,8 +540,8 @@ k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, i
     const int o = bid / nlt, lt = bid % nlt;
     const int l = lt * P + p;
     const bool lv = l < Mz;
-    const long S = AXIS == 0 ? (long)Ny * Mz : (long)Mz;
-    const long base = AXIS == 0 ? (long)o * Mz + l : (long)o * Ny * Mz + l;
+    const long S = AXIS == 0 ? (long)Ny * Ms : (long)Ms;
+    const long base = AXIS == 0 ? (long)o * Ms + l : (long)o * Ny * Ms + l;
 #pragma unroll 1
     NS3D_FIELD_LOOP(f, nf) {
         const cplx<TB>* src = in + (long)f * nc;
@@ -671,7 +673,7 @@ __device__ __forceinline__ void ipq_rule(cplx<double>& Qx, cplx<double>& Qy, cpl
 
 template <typename TB, int NX, int P, int MODE, int E>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
-k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int nlt, int nb, SpecTables tab,
+k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int nlt, int nb, SpecTables tab,
             const cplx<TB>* __restrict__ twg, EpiArgs ea) {
     constexpr int TPP = XYPlan<NX, E>::TPP;
     using F = PencilFFT<TB, NX, TPP, -1, typename XYPlan<NX, E>::R, IndexMajor<P>>;
@@ -684,8 +686,10 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int nlt, in
     const int k = bid / nlt, lt = bid % nlt;
     const int l = lt * P + p;
     const bool lv = l < Mz;
-    const long S = (long)Ny * Mz;
+    const long S = (long)Ny * Mz;        // state / output: reference layout
     const long base = (long)k * Mz + l;
+    const long Si = (long)Ny * Ms;       // D: rows padded to Ms
+    const long basei = (long)k * Ms + l;
     cplx<TB> qv[3][F::E];
     // the three components are all live until the epilogue: load them together
 #pragma unroll
@@ -696,7 +700,7 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int nlt, in
 #pragma unroll
             for (int r = 0; r < F::R0; r++) {
                 cplx<TB> s = mk<TB>(0, 0);
-                if (F::in_active(t, q) && lv) s = src[base + F::in_index(t, q, r) * S];
+                if (F::in_active(t, q) && lv) s = src[basei + F::in_index(t, q, r) * Si];
                 qv[b][q * F::R0 + r] = s;
             }
         }

@@ -72,6 +72,7 @@ template <int N> constexpr int PZ = (128 / ZPlan<N>::TPP) > 0 ? (128 / ZPlan<N>:
 struct ns3d_ctx {
     int Nx, Ny, Nz, Mz;
     long nc, nr, npen;
+    long nci;  // modes per scratch field of the fused passes (rows padded from Mz to mi_of(c))
     double Lx, Ly, Lz, Re, dt;
     int precision;
     unsigned flags;
@@ -260,6 +261,16 @@ template <typename TB> const cplx<TB>* twp(const ns3d_ctx* c, int axis) {
 
 size_t bulk_bytes(const ns3d_ctx* c) { return c->precision == NS3D_FP64 ? sizeof(zd) : sizeof(cplx<float>); }
 
+// Row stride of the fused passes' scratch fields: Mz rounded up so that every
+// P-mode row tile of the x/y passes (128 B) starts on a 128-B line.
+#ifndef NS3D_PAD_ROWS
+#define NS3D_PAD_ROWS 1
+#endif
+int mi_of(const ns3d_ctx* c) {
+    const int a = NS3D_PAD_ROWS ? (c->precision == NS3D_FP64 ? 8 : 16) : 1;
+    return (c->Mz + a - 1) / a * a;
+}
+
 // Forget a captured RK3 graph (its nodes hold scratch pointers, table values and the stream).
 void drop_graph(ns3d_ctx* c) {
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
@@ -278,7 +289,7 @@ int ensure_scratch(ns3d_ctx* c, int fields) {
         c->scratch = nullptr;
         c->scratch_fields = 0;
     }
-    if (hipMalloc(&c->scratch, bulk_bytes(c) * (size_t)c->nc * fields) != hipSuccess) return NS3D_ERR_ALLOC;
+    if (hipMalloc(&c->scratch, bulk_bytes(c) * (size_t)c->nci * fields) != hipSuccess) return NS3D_ERR_ALLOC;
     c->scratch_fields = fields;
     return NS3D_OK;
 }
@@ -300,14 +311,16 @@ inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
 inline dim3 xgrid(long nb, int nfy = 1) { return dim3((unsigned)(((nb + 7) / 8) * 8), (unsigned)nfy); }
 
 // ---------------------------------------------------------------- fused pipeline
-// Runs P1..P4 for NF vector fields and leaves D (3 fields) at scratch + 3*nc.
+// Runs P1..P4 for NF vector fields and leaves D (3 fields) at scratch + 3*nci.
 template <typename TB, typename TS = double>
 int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf) {
     int st = ensure_scratch(c, 15 * nf);
     if (st) return st;
+    // scratch fields: rows padded to Mi (128-B aligned pencil tiles), nci modes each
+    const long nc = c->nci;
+    const int Mi = mi_of(c);
     cplx<TB>* A = (cplx<TB>*)c->scratch;
-    cplx<TB>* B = A + (long)6 * nf * c->nc;
-    const long nc = c->nc;
+    cplx<TB>* B = A + (long)6 * nf * nc;
     hipStream_t s = c->stream;
     // P1 x-inverse
     prof_begin(c, 0);
@@ -321,11 +334,12 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf) {
         dim3 grid = xgrid(nb, 3 * nf * (OS ? 2 : 1)), blk(P * XYPlan<NX, E>::TPP);
         if (nf == 1) {
             PtrArr<const cplx<TS>, 3> w{{in[0], in[1], in[2]}};
-            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E, OS, TS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab,
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E, OS, TS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, Mi, nlt, nb, c->tab,
                                twp<TB>(c, 0));
         } else if constexpr (std::is_same<TS, double>::value) {
             PtrArr<const zd, 6> w{{in[0], in[1], in[2], in[3], in[4], in[5]}};
-            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2, E, OS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 0));
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2, E, OS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, Mi, nlt, nb, c->tab,
+                               twp<TB>(c, 0));
         } else {
             return NS3D_ERR_ARG;
         }
@@ -344,9 +358,11 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf) {
         constexpr int OS = NS3D_OS_YINV;
         dim3 grid = xgrid(nb, 3 * nf * (OS ? 3 : 1)), blk(P * XYPlan<NY, E>::TPP);
         if (nf == 1)
-            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 1, E, OS>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 1));
+            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 1, E, OS>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, Mi, nlt, nb, c->tab,
+                               twp<TB>(c, 1));
         else
-            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 2, E, OS>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, nlt, nb, c->tab, twp<TB>(c, 1));
+            hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 2, E, OS>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, Mi, nlt, nb, c->tab,
+                               twp<TB>(c, 1));
         return NS3D_OK;
     });
     prof_end(c);
@@ -361,10 +377,10 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf) {
         const TB scale = (TB)(1.0 / (1.0 * c->Nx * c->Ny * c->Nz));
         TB* up = nf == 1 ? (TB*)c->uphys_out : nullptr;
         if (nf == 1)
-            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 1>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, nb, c->tab, twp<TB>(c, 2), scale,
+            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 1>), grid, blk, 0, s, B, A, nc, c->npen, Mi, nb, c->tab, twp<TB>(c, 2), scale,
                                up, c->nr);
         else
-            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 2>), grid, blk, 0, s, B, A, nc, c->npen, c->Mz, nb, c->tab, twp<TB>(c, 2), scale,
+            hipLaunchKernelGGL((k_z_product<TB, NZ, P, 2>), grid, blk, 0, s, B, A, nc, c->npen, Mi, nb, c->tab, twp<TB>(c, 2), scale,
                                (TB*)nullptr, c->nr);
         return NS3D_OK;
     });
@@ -379,7 +395,8 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf) {
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Nx * nlt;
         dim3 grid = xgrid(nb, 3), blk(P * XYPlan<NY, E>::TPP);
-        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1, E>), grid, blk, 0, s, A, A + 3 * nc, 3, nc, c->Ny, c->Mz, nlt, nb, twp<TB>(c, 1));
+        hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1, E>), grid, blk, 0, s, A, A + 3 * nc, 3, nc, c->Ny, c->Mz, Mi, nlt, nb,
+                           twp<TB>(c, 1));
         return NS3D_OK;
     });
     prof_end(c);
@@ -388,7 +405,7 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf) {
 
 template <typename TB, int MODE>
 int run_x_forward_epilogue(ns3d_ctx* c, const EpiArgs& ea) {
-    const cplx<TB>* D = (const cplx<TB>*)c->scratch + 3 * c->nc;
+    const cplx<TB>* D = (const cplx<TB>*)c->scratch + 3 * c->nci;
     prof_begin(c, 4);
     int st = with_size(c->Nx, [&](auto NXc) -> int {
         constexpr int NX = decltype(NXc)::value;
@@ -397,7 +414,8 @@ int run_x_forward_epilogue(ns3d_ctx* c, const EpiArgs& ea) {
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Ny * nlt;
         dim3 grid = xgrid(nb), blk(P * XYPlan<NX, E>::TPP);
-        hipLaunchKernelGGL((k_x_fwd_epi<TB, NX, P, MODE, E>), grid, blk, 0, c->stream, D, c->nc, c->Ny, c->Mz, nlt, nb, c->tab,
+        hipLaunchKernelGGL((k_x_fwd_epi<TB, NX, P, MODE, E>), grid, blk, 0, c->stream, D, c->nci, c->Ny, c->Mz, mi_of(c), nlt, nb,
+                           c->tab,
                            twp<TB>(c, 0), ea);
         return NS3D_OK;
     });
@@ -545,7 +563,7 @@ int c2r_3d(ns3d_ctx* c, const cplx<TB>* in, TB* out, cplx<TB>* tmp, bool scale) 
         constexpr int P = PXY<TB, NX, E>;
         const int nlt = (c->Mz + P - 1) / P;
         hipLaunchKernelGGL((k_axis_c2c<TB, NX, P, +1, 0, E>), xgrid(c->Ny * nlt), dim3(P * XYPlan<NX, E>::TPP), 0, s, in, tmp, 1, nc,
-                           c->Ny, c->Mz, nlt, c->Ny * nlt, twp<TB>(c, 0));
+                           c->Ny, c->Mz, c->Mz, nlt, c->Ny * nlt, twp<TB>(c, 0));
         return NS3D_OK;
     });
     if (st) return st;
@@ -555,7 +573,7 @@ int c2r_3d(ns3d_ctx* c, const cplx<TB>* in, TB* out, cplx<TB>* tmp, bool scale) 
         constexpr int P = PXY<TB, NY, E>;
         const int nlt = (c->Mz + P - 1) / P;
         hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, +1, 1, E>), xgrid(c->Nx * nlt), dim3(P * XYPlan<NY, E>::TPP), 0, s, tmp, tmp, 1,
-                           nc, c->Ny, c->Mz, nlt, c->Nx * nlt, twp<TB>(c, 1));
+                           nc, c->Ny, c->Mz, c->Mz, nlt, c->Nx * nlt, twp<TB>(c, 1));
         return NS3D_OK;
     });
     if (st) return st;
@@ -591,7 +609,7 @@ int r2c_3d(ns3d_ctx* c, const TB* in, cplx<TB>* out) {
         constexpr int P = PXY<TB, NY, E>;
         const int nlt = (c->Mz + P - 1) / P;
         hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1, E>), xgrid(c->Nx * nlt), dim3(P * XYPlan<NY, E>::TPP), 0, s, out, out, 1,
-                           nc, c->Ny, c->Mz, nlt, c->Nx * nlt, twp<TB>(c, 1));
+                           nc, c->Ny, c->Mz, c->Mz, nlt, c->Nx * nlt, twp<TB>(c, 1));
         return NS3D_OK;
     });
     if (st) return st;
@@ -601,7 +619,7 @@ int r2c_3d(ns3d_ctx* c, const TB* in, cplx<TB>* out) {
         constexpr int P = PXY<TB, NX, E>;
         const int nlt = (c->Mz + P - 1) / P;
         hipLaunchKernelGGL((k_axis_c2c<TB, NX, P, -1, 0, E>), xgrid(c->Ny * nlt), dim3(P * XYPlan<NX, E>::TPP), 0, s, out, out, 1,
-                           nc, c->Ny, c->Mz, nlt, c->Ny * nlt, twp<TB>(c, 0));
+                           nc, c->Ny, c->Mz, c->Mz, nlt, c->Ny * nlt, twp<TB>(c, 0));
         return NS3D_OK;
     });
 }
@@ -755,6 +773,7 @@ int ns3d_create(const ns3d_config* cfg, ns3d_ctx** out) {
     c->dt = cfg->dt;
     c->precision = cfg->precision;
     c->flags = cfg->flags;
+    c->nci = (long)c->Nx * c->Ny * mi_of(c);
     c->stream = (hipStream_t)cfg->stream;
     hipGetDevice(&c->device);
     c->last_error = NS3D_OK;
@@ -818,7 +837,7 @@ int ns3d_last_error(const ns3d_ctx* c) { return c ? c->last_error : NS3D_ERR_ARG
 
 long long ns3d_workspace_bytes(const ns3d_ctx* c) {
     if (!c) return -1;
-    long long b = (long long)bulk_bytes(c) * c->nc * c->scratch_fields;
+    long long b = (long long)bulk_bytes(c) * c->nci * c->scratch_fields;
     if (c->stage) b += (long long)sizeof(zd) * c->nc * 9;
     if (c->extra) b += (long long)sizeof(zd) * c->nc * 6;
     if (c->kry) b += (long long)sizeof(zd) * c->nc * 3;
