@@ -45,8 +45,8 @@ template <int N, int E> using XYPlan = typename XYSel<N, E>::type;
 // TS: scalar type of the state w (double, or float for the FP32-state RK3 loop).
 template <typename TB, int NX, int P, int NF, int E, int OS, typename TS = double>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
-k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int Ms, int nlt,
-             int nb, SpecTables tab, const cplx<TB>* __restrict__ twg) {
+k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int Mw, int Ms,
+             int nlt, int nb, SpecTables tab, const cplx<TB>* __restrict__ twg) {
     constexpr int TPP = XYPlan<NX, E>::TPP;
     using F = PencilFFT<TB, NX, TPP, +1, typename XYPlan<NX, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NX * P];
@@ -61,8 +61,8 @@ k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc
     const bool keep = lv && column_kept(tab, k, l);
     if (!__syncthreads_or(keep)) return;
     stage_twiddles<TB, NX>(tw, twg);
-    const long S = (long)Ny * Mz;        // state: reference layout
-    const long base = (long)k * Mz + l;
+    const long S = (long)Ny * Mw;        // state: row stride Mw (Mz: reference layout)
+    const long base = (long)k * Mw + l;
     const long So = (long)Ny * Ms;       // scratch: rows padded to Ms
     const long baseo = (long)k * Ms + l;
     const int f = OS ? (int)blockIdx.y >> 1 : (int)blockIdx.y;
@@ -581,16 +581,21 @@ struct EpiArgs {
     cplx<double>* out[3];        // output (EPI_Q, EPI_STAGE, EPI_RHS); cplx<float> for EPI_STAGE32
     double dt, Re, a0, a1;       // EPI_STAGE: out = a0*u0 + a1*E(w)
     int ipq;                     // reference IPQ rule (Q1)
+    // mode (j,k,l) of w / u0 / out / f sits at (j*ny + k)*m + l: m = Mz for the
+    // reference layout, a padded row stride for the library's own stage buffers
+    int ny, mw, mu0, mo, mf;
 };
 
 // Projection/diffusion/update at one mode (Source/diffusion.cu:96-365).
 template <int MODE>
-__device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& tab, int j, int k, int l, long g,
+__device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& tab, int j, int k, int l,
                                          cplx<double> Qx, cplx<double> Qy, cplx<double> Qz) {
+    const long row = (long)j * ea.ny + k;
+    const long gw = row * ea.mw + l, gu = row * ea.mu0 + l, go = row * ea.mo + l, gf = row * ea.mf + l;
     if (MODE == EPI_Q) {
-        ea.out[0][g] = Qx;
-        ea.out[1][g] = Qy;
-        ea.out[2][g] = Qz;
+        ea.out[0][go] = Qx;
+        ea.out[1][go] = Qy;
+        ea.out[2][go] = Qz;
         return;
     }
     const ProjCoef c = proj_coef(tab, j, k, l);
@@ -604,55 +609,55 @@ __device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& ta
     for (int b = 0; b < 3; b++) {
         cplx<double> r;
         if (MODE == EPI_STAGE32) {
-            const cplx<double> w = cvt<double>(reinterpret_cast<const cplx<float>*>(ea.w[b])[g]);
+            const cplx<double> w = cvt<double>(reinterpret_cast<const cplx<float>*>(ea.w[b])[gw]);
             const double G = (1.0 - c.din * dt / Re);
             cplx<double> f = mk<double>(0.0, 0.0);
-            if (ea.f[b]) f = ea.f[b][g];
+            if (ea.f[b]) f = ea.f[b][gf];
             r.x = dt * AQr[b] + G * w.x + dt * f.x - dt * Q[b].x;
             r.y = dt * AQi[b] + G * w.y + dt * f.y - dt * Q[b].y;
             if (ea.u0[b]) {
-                const cplx<double> u0 = cvt<double>(reinterpret_cast<const cplx<float>*>(ea.u0[b])[g]);
+                const cplx<double> u0 = cvt<double>(reinterpret_cast<const cplx<float>*>(ea.u0[b])[gu]);
                 r.x = ea.a0 * u0.x + ea.a1 * r.x;
                 r.y = ea.a0 * u0.y + ea.a1 * r.y;
             }
-            reinterpret_cast<cplx<float>*>(ea.out[b])[g] = cvt<float>(r);
+            reinterpret_cast<cplx<float>*>(ea.out[b])[go] = cvt<float>(r);
             continue;
         }
-        const cplx<double> w = ea.w[b][g];
+        const cplx<double> w = ea.w[b][gw];
         if (MODE == EPI_STAGE) {
             const double G = (1.0 - c.din * dt / Re);
             cplx<double> f = mk<double>(0.0, 0.0);
-            if (ea.f[b]) f = ea.f[b][g];
+            if (ea.f[b]) f = ea.f[b][gf];
             r.x = dt * AQr[b] + G * w.x + dt * f.x - dt * Q[b].x;
             r.y = dt * AQi[b] + G * w.y + dt * f.y - dt * Q[b].y;
             if (ea.u0[b]) {
-                const cplx<double> u0 = ea.u0[b][g];
+                const cplx<double> u0 = ea.u0[b][gu];
                 r.x = ea.a0 * u0.x + ea.a1 * r.x;
                 r.y = ea.a0 * u0.y + ea.a1 * r.y;
             }
-            ea.out[b][g] = r;
+            ea.out[b][go] = r;
         } else if (MODE == EPI_RHS) {
             const double G = (-c.din / Re);
             cplx<double> f = mk<double>(0.0, 0.0);
-            if (ea.f[b]) f = ea.f[b][g];
+            if (ea.f[b]) f = ea.f[b][gf];
             r.x = AQr[b] + G * w.x + f.x - Q[b].x;
             r.y = AQi[b] + G * w.y + f.y - Q[b].y;
-            ea.out[b][g] = r;
+            ea.out[b][go] = r;
         } else if (MODE == EPI_UV) {
             const double G = (1.0 - c.din * dt / Re);
             r.x = dt * AQr[b] + G * w.x - dt * Q[b].x;
             r.y = dt * AQi[b] + G * w.y - dt * Q[b].y;
-            ea.w[b][g] = r;
+            ea.w[b][gw] = r;
         } else if (MODE == EPI_UV_RHS) {
             const double G = (-c.din / Re);
             r.x = AQr[b] + G * w.x - Q[b].x;
             r.y = AQi[b] + G * w.y - Q[b].y;
-            ea.w[b][g] = r;
+            ea.w[b][gw] = r;
         } else {  // EPI_IUV
             const double G = (1.0 + c.din * dt / Re);
             r.x = (dt * AQr[b] + w.x - dt * Q[b].x) / G;
             r.y = (dt * AQi[b] + w.y - dt * Q[b].y) / G;
-            ea.w[b][g] = r;
+            ea.w[b][gw] = r;
         }
     }
 }
@@ -686,8 +691,6 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int
     const int k = bid / nlt, lt = bid % nlt;
     const int l = lt * P + p;
     const bool lv = l < Mz;
-    const long S = (long)Ny * Mz;        // state / output: reference layout
-    const long base = (long)k * Mz + l;
     const long Si = (long)Ny * Ms;       // D: rows padded to Ms
     const long basei = (long)k * Ms + l;
     cplx<TB> qv[3][F::E];
@@ -717,12 +720,11 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int
 #pragma unroll
         for (int r = 0; r < F::RL; r++) {
             const int j = F::out_index(t, q, r);
-            const long g = base + (long)j * S;
             cplx<double> Qx = cvt<double>(qv[0][q * F::RL + r]);
             cplx<double> Qy = cvt<double>(qv[1][q * F::RL + r]);
             cplx<double> Qz = cvt<double>(qv[2][q * F::RL + r]);
             ipq_rule(Qx, Qy, Qz, ea.ipq, (j | k | l) == 0);
-            epilogue<MODE>(ea, tab, j, k, l, g, Qx, Qy, Qz);
+            epilogue<MODE>(ea, tab, j, k, l, Qx, Qy, Qz);
         }
     }
 }
@@ -737,7 +739,7 @@ k_update(const cplx<double>* __restrict__ Qx, const cplx<double>* __restrict__ Q
     const int l = (int)(g % Mz);
     const long t1 = g / Mz;
     const int k = (int)(t1 % Ny), j = (int)(t1 / Ny);
-    epilogue<MODE>(ea, tab, j, k, l, g, Qx[g], Qy[g], Qz[g]);
+    epilogue<MODE>(ea, tab, j, k, l, Qx[g], Qy[g], Qz[g]);
 }
 
 }  // namespace ns3d

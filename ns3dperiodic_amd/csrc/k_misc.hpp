@@ -80,6 +80,19 @@ __global__ void k_f2d_c(const cplx<float>* __restrict__ in, cplx<double>* __rest
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = cvt<double>(in[i]);
 }
+// FP64 field in the reference layout (rows of Mz) <-> FP32 field with rows padded to Mp
+__global__ void k_d2f_rows(const cplx<double>* __restrict__ in, cplx<float>* __restrict__ out, long n, int Mz, int Mp) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long row = i / Mz;
+    out[row * Mp + (i - row * Mz)] = cvt<float>(in[i]);
+}
+__global__ void k_f2d_rows(const cplx<float>* __restrict__ in, cplx<double>* __restrict__ out, long n, int Mz, int Mp) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long row = i / Mz;
+    out[i] = cvt<double>(in[row * Mp + (i - row * Mz)]);
+}
 
 // ------------------------------------------------------- a7: copy / axpby on 3 fields
 // single_RK_step_device (Source/RK_time_step.cu:82-115): dst = w0*s0 + w1*s1.

@@ -73,6 +73,8 @@ struct ns3d_ctx {
     int Nx, Ny, Nz, Mz;
     long nc, nr, npen;
     long nci;  // modes per scratch field of the fused passes (rows padded from Mz to mi_of(c))
+    int m8, m16;     // Mz rounded up to a multiple of 8 / 16 (128-B rows for FP64 / FP32 complex)
+    long nc8, nc16;  // Nx*Ny*m8, Nx*Ny*m16
     double Lx, Ly, Lz, Re, dt;
     int precision;
     unsigned flags;
@@ -302,7 +304,8 @@ int ensure_real_scratch(ns3d_ctx* c) {
 
 int ensure_fp64(ns3d_ctx* c, zd** p, int fields) {
     if (*p) return NS3D_OK;
-    if (hipMalloc((void**)p, sizeof(zd) * (size_t)c->nc * fields) != hipSuccess) return NS3D_ERR_ALLOC;
+    // nc8 >= nc: room for fields in either the reference or the padded layout
+    if (hipMalloc((void**)p, sizeof(zd) * (size_t)c->nc8 * fields) != hipSuccess) return NS3D_ERR_ALLOC;
     return NS3D_OK;
 }
 
@@ -312,8 +315,10 @@ inline dim3 xgrid(long nb, int nfy = 1) { return dim3((unsigned)(((nb + 7) / 8) 
 
 // ---------------------------------------------------------------- fused pipeline
 // Runs P1..P4 for NF vector fields and leaves D (3 fields) at scratch + 3*nci.
+// mw: row stride of the input fields (0: Mz, the reference layout)
 template <typename TB, typename TS = double>
-int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf) {
+int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw = 0) {
+    if (mw == 0) mw = c->Mz;
     int st = ensure_scratch(c, 15 * nf);
     if (st) return st;
     // scratch fields: rows padded to Mi (128-B aligned pencil tiles), nci modes each
@@ -334,11 +339,11 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf) {
         dim3 grid = xgrid(nb, 3 * nf * (OS ? 2 : 1)), blk(P * XYPlan<NX, E>::TPP);
         if (nf == 1) {
             PtrArr<const cplx<TS>, 3> w{{in[0], in[1], in[2]}};
-            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E, OS, TS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, Mi, nlt, nb, c->tab,
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E, OS, TS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, mw, Mi, nlt, nb, c->tab,
                                twp<TB>(c, 0));
         } else if constexpr (std::is_same<TS, double>::value) {
             PtrArr<const zd, 6> w{{in[0], in[1], in[2], in[3], in[4], in[5]}};
-            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2, E, OS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, Mi, nlt, nb, c->tab,
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2, E, OS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, mw, Mi, nlt, nb, c->tab,
                                twp<TB>(c, 0));
         } else {
             return NS3D_ERR_ARG;
@@ -431,6 +436,8 @@ EpiArgs make_epi(const ns3d_ctx* c) {
     ea.a0 = 0.0;
     ea.a1 = 1.0;
     ea.ipq = (c->flags & NS3D_FLAG_NO_IPQ) ? 0 : 1;
+    ea.ny = c->Ny;
+    ea.mw = ea.mu0 = ea.mo = ea.mf = c->Mz;
     return ea;
 }
 
@@ -439,11 +446,11 @@ template <int MODE>
 int fused(ns3d_ctx* c, const zd* const* in, int nf, const EpiArgs& ea) {
     int st;
     if (c->precision == NS3D_FP64) {
-        st = run_nonlinear_passes<double>(c, in, nf);
+        st = run_nonlinear_passes<double>(c, in, nf, ea.mw);
         if (st) return st;
         st = run_x_forward_epilogue<double, MODE>(c, ea);
     } else {
-        st = run_nonlinear_passes<float>(c, in, nf);
+        st = run_nonlinear_passes<float>(c, in, nf, ea.mw);
         if (st) return st;
         st = run_x_forward_epilogue<float, MODE>(c, ea);
     }
@@ -451,10 +458,13 @@ int fused(ns3d_ctx* c, const zd* const* in, int nf, const EpiArgs& ea) {
     return hipGetLastError() == hipSuccess ? NS3D_OK : NS3D_ERR_HIP;
 }
 
-// E(w) stage: out = a0*u0 + a1*(w + dt*L(w))
+// E(w) stage: out = a0*u0 + a1*(w + dt*L(w)); mw/mu0/mo: row strides (0: Mz)
 int stage_into(ns3d_ctx* c, const zd* const w[3], const zd* const u0[3], double a0, double a1, zd* const out[3],
-               const zd* const f[3]) {
+               const zd* const f[3], int mw = 0, int mu0 = 0, int mo = 0) {
     EpiArgs ea = make_epi(c);
+    if (mw) ea.mw = mw;
+    if (mu0) ea.mu0 = mu0;
+    if (mo) ea.mo = mo;
     for (int b = 0; b < 3; b++) {
         ea.w[b] = (zd*)w[b];
         ea.u0[b] = u0 ? u0[b] : nullptr;
@@ -469,16 +479,20 @@ int stage_into(ns3d_ctx* c, const zd* const w[3], const zd* const u0[3], double 
 int rk3_one(ns3d_ctx* c, zd* const u[3], const zd* const f[3]) {
     int st = ensure_fp64(c, &c->stage, 9);
     if (st) return st;
-    zd* u1[3] = {c->stage, c->stage + c->nc, c->stage + 2 * c->nc};
-    zd* u2[3] = {c->stage + 3 * c->nc, c->stage + 4 * c->nc, c->stage + 5 * c->nc};
+    // u1, u2 are the library's own: rows padded to M8 (128-B aligned), fields nc8 apart
+    const long n8 = c->nc8;
+    const int m8 = c->m8;
+    zd* u1[3] = {c->stage, c->stage + n8, c->stage + 2 * n8};
+    zd* u2[3] = {c->stage + 3 * n8, c->stage + 4 * n8, c->stage + 5 * n8};
     const zd* uc[3] = {u[0], u[1], u[2]};
     const zd* u1c[3] = {u1[0], u1[1], u1[2]};
     const zd* u2c[3] = {u2[0], u2[1], u2[2]};
+    const int mz = c->Mz;
     // RK3_SSP (Source/RK_time_step.cu:310-327):
     //   u1 = E(u); u1 = E(u1); u2 = 3/4 u + 1/4 u1; u2 = E(u2); u = 1/3 u + 2/3 u2
-    if ((st = stage_into(c, uc, nullptr, 0.0, 1.0, u1, f))) return st;
-    if ((st = stage_into(c, u1c, uc, 3.0 / 4.0, 1.0 / 4.0, u2, f))) return st;
-    if ((st = stage_into(c, u2c, uc, 1.0 / 3.0, 2.0 / 3.0, u, f))) return st;
+    if ((st = stage_into(c, uc, nullptr, 0.0, 1.0, u1, f, mz, mz, m8))) return st;
+    if ((st = stage_into(c, u1c, uc, 3.0 / 4.0, 1.0 / 4.0, u2, f, m8, mz, m8))) return st;
+    if ((st = stage_into(c, u2c, uc, 1.0 / 3.0, 2.0 / 3.0, u, f, m8, mz, mz))) return st;
     return NS3D_OK;
 }
 
@@ -498,7 +512,8 @@ int stage_into32(ns3d_ctx* c, const cf* const w[3], const cf* const u0[3], doubl
     }
     ea.a0 = a0;
     ea.a1 = a1;
-    int st = run_nonlinear_passes<float, float>(c, w, 1);
+    ea.mw = ea.mu0 = ea.mo = c->m16;  // the FP32 state and stages use rows padded to m16
+    int st = run_nonlinear_passes<float, float>(c, w, 1, c->m16);
     if (st) return st;
     st = run_x_forward_epilogue<float, EPI_STAGE32>(c, ea);
     if (st) return st;
@@ -506,7 +521,7 @@ int stage_into32(ns3d_ctx* c, const cf* const w[3], const cf* const u0[3], doubl
 }
 
 int rk3_one32(ns3d_ctx* c, const zd* const f[3]) {
-    const long nc = c->nc;
+    const long nc = c->nc16;
     cf* u[3] = {c->s32, c->s32 + nc, c->s32 + 2 * nc};
     cf* u1[3] = {c->s32 + 3 * nc, c->s32 + 4 * nc, c->s32 + 5 * nc};
     cf* u2[3] = {c->s32 + 6 * nc, c->s32 + 7 * nc, c->s32 + 8 * nc};
@@ -522,7 +537,7 @@ int rk3_one32(ns3d_ctx* c, const zd* const f[3]) {
 
 int ensure_state32(ns3d_ctx* c) {
     if (c->s32) return NS3D_OK;
-    if (hipMalloc((void**)&c->s32, sizeof(cf) * (size_t)c->nc * 9) != hipSuccess) return NS3D_ERR_ALLOC;
+    if (hipMalloc((void**)&c->s32, sizeof(cf) * (size_t)c->nc16 * 9) != hipSuccess) return NS3D_ERR_ALLOC;
     return NS3D_OK;
 }
 
@@ -539,15 +554,17 @@ int rk3_steps(ns3d_ctx* c, int nsteps, zd* const u[3], const zd* const f[3]) {
     int st = ensure_state32(c);
     if (!st) st = ensure_scratch(c, 15);
     if (st) return st;
-    const long nc = c->nc;
+    const long nc = c->nc, n16 = c->nc16;
     for (int b = 0; b < 3; b++)
-        hipLaunchKernelGGL(k_d2f_c, dim3(nblk(nc, 256)), dim3(256), 0, c->stream, u[b], c->s32 + b * nc, nc);
+        hipLaunchKernelGGL(k_d2f_rows, dim3(nblk(nc, 256)), dim3(256), 0, c->stream, u[b], c->s32 + b * n16, nc, c->Mz,
+                           c->m16);
     for (int i = 0; i < nsteps; i++) {
         st = rk3_one32(c, f);
         if (st) return st;
     }
     for (int b = 0; b < 3; b++)
-        hipLaunchKernelGGL(k_f2d_c, dim3(nblk(nc, 256)), dim3(256), 0, c->stream, c->s32 + b * nc, u[b], nc);
+        hipLaunchKernelGGL(k_f2d_rows, dim3(nblk(nc, 256)), dim3(256), 0, c->stream, c->s32 + b * n16, u[b], nc, c->Mz,
+                           c->m16);
     HIPCHK(hipGetLastError());
     return NS3D_OK;
 }
@@ -774,6 +791,10 @@ int ns3d_create(const ns3d_config* cfg, ns3d_ctx** out) {
     c->precision = cfg->precision;
     c->flags = cfg->flags;
     c->nci = (long)c->Nx * c->Ny * mi_of(c);
+    c->m8 = NS3D_PAD_ROWS ? (c->Mz + 7) / 8 * 8 : c->Mz;
+    c->m16 = NS3D_PAD_ROWS ? (c->Mz + 15) / 16 * 16 : c->Mz;
+    c->nc8 = (long)c->Nx * c->Ny * c->m8;
+    c->nc16 = (long)c->Nx * c->Ny * c->m16;
     c->stream = (hipStream_t)cfg->stream;
     hipGetDevice(&c->device);
     c->last_error = NS3D_OK;
@@ -838,10 +859,10 @@ int ns3d_last_error(const ns3d_ctx* c) { return c ? c->last_error : NS3D_ERR_ARG
 long long ns3d_workspace_bytes(const ns3d_ctx* c) {
     if (!c) return -1;
     long long b = (long long)bulk_bytes(c) * c->nci * c->scratch_fields;
-    if (c->stage) b += (long long)sizeof(zd) * c->nc * 9;
-    if (c->extra) b += (long long)sizeof(zd) * c->nc * 6;
-    if (c->kry) b += (long long)sizeof(zd) * c->nc * 3;
-    if (c->s32) b += (long long)sizeof(cplx<float>) * c->nc * 9;
+    if (c->stage) b += (long long)sizeof(zd) * c->nc8 * 9;
+    if (c->extra) b += (long long)sizeof(zd) * c->nc8 * 6;
+    if (c->kry) b += (long long)sizeof(zd) * c->nc8 * 3;
+    if (c->s32) b += (long long)sizeof(cplx<float>) * c->nc16 * 9;
     if (c->real_scratch) b += (long long)sizeof(double) * c->nr * 4;
     return b;
 }
