@@ -26,24 +26,41 @@ PASS_NAMES = ["x_inverse", "y_inverse", "z_product", "y_forward", "x_forward_epi
 PASS_KERNELS = ["k_x_inv_grad", "k_y_inv_grad", "k_z_product", "k_axis_c2c", "k_x_fwd_epi"]
 
 
-def pass_bytes(nc, bulk_bytes):
+def pass_bytes(nc, bulk_bytes, state_bytes=16):
     """Algorithmic HBM bytes per launch of each pass of one RK3 stage.
 
     Field traffic of the five-pass dataflow (S = one spectral field of nc
     modes): P1 reads 3 state fields and writes 6; P2 reads 6, writes 9; P3
     reads 9, writes 3; P4 reads 3, writes 3; P5 reads 3 bulk + 3 state (+3
     u^n in stages 2-3) and writes 3 state.  State fields are FP64 complex
-    (16 B); bulk fields use the bulk precision.  With FP64 bulk the RK3 step
-    totals 159 S, the algorithmic byte count of the path.
+    (16 B) except in the FP32-state loop (8 B); bulk fields use the bulk
+    precision.  With one precision throughout the RK3 step totals 159 S, the
+    algorithmic byte count of the path.
     """
-    s, b = 16 * nc, bulk_bytes * nc
+    s, b = state_bytes * nc, bulk_bytes * nc
     p5_avg = 3 * b + 3 * s + 3 * s + (2.0 / 3.0) * 3 * s
     return [3 * s + 6 * b, 15 * b, 12 * b, 6 * b, p5_avg]
 
 
-def step_bytes(nc, bulk_bytes):
-    pb = pass_bytes(nc, bulk_bytes)
+def step_bytes(nc, bulk_bytes, state_bytes=16):
+    pb = pass_bytes(nc, bulk_bytes, state_bytes)
     return 3 * sum(pb)
+
+
+def profiled_traffic(precision, kernel_prefix):
+    """HBM bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+    of a kernel at 256^3, from the committed profile summary; None if absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "rk3_256_passes_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        line = line.strip()
+        if not line.startswith("{"):
+            continue
+        d = json.loads(line)
+        if d["precision"] == precision and d["kernel"].startswith(kernel_prefix + "<"):
+            return d["fetch_bytes"] + d["write_bytes"]
+    return None
 
 
 def run_gpu(N, precision, steps, warmup, u0, Re, dt):
@@ -74,8 +91,9 @@ def run_gpu(N, precision, steps, warmup, u0, Re, dt):
     finite = bool(torch.isfinite(u[0]).all().item())
     g.close()
     bulk = 16 if precision == "fp64" else 8
+    state = 8 if precision == "fp32s" else 16
     nc = N * N * (N // 2 + 1)
-    pb = pass_bytes(nc, bulk)
+    pb = pass_bytes(nc, bulk, state)
     passes = []
     for i in range(5):
         avg = ms[i] / calls[i] if calls[i] else float("nan")
@@ -83,7 +101,7 @@ def run_gpu(N, precision, steps, warmup, u0, Re, dt):
         passes.append({"pass": PASS_NAMES[i], "kernel": PASS_KERNELS[i], "calls": calls[i], "avg_ms": avg,
                        "total_ms": ms[i], "alg_bytes": pb[i], "GBps": gbs})
     ms_step = total_ms / steps
-    sb = step_bytes(nc, bulk)
+    sb = step_bytes(nc, bulk, state)
     return {
         "precision": precision,
         "steps_per_s": steps / (total_ms * 1e-3),
@@ -238,8 +256,11 @@ def main():
         print(f"[bench] {p}: {secs[-1]['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
 
     dom = max(prim["passes"], key=lambda p: p["total_ms"])
+    traffic = profiled_traffic(args.precision, dom["kernel"]) if N == 256 else None
     roofline = {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
+                "traffic_source": "profiles/rk3_256_passes_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, "
+                                  "scaled by a plain-copy calibration)" if traffic else None,
                 "alg_bytes_per_launch": dom["alg_bytes"], "avg_launch_ms": dom["avg_ms"]}
 
     cpu = None

@@ -42,13 +42,22 @@ bool size_ok(int n) {
 }
 
 // pencils per block in the x/y passes: rows of 128 B
-template <typename TB> constexpr int PXY0 = sizeof(TB) == 8 ? 8 : 16;
+#ifndef NS3D_PXY32
+#define NS3D_PXY32 16
+#endif
+template <typename TB> constexpr int PXY0 = sizeof(TB) == 8 ? 8 : NS3D_PXY32;
 template <typename TB, int N, int E>
 constexpr int PXY = PXY0<TB> * XYPlan<N, E>::TPP <= 1024 ? PXY0<TB> : 1024 / XYPlan<N, E>::TPP;
 // Points per thread of each x/y pass, by precision (measured at 256^3 on MI355X):
 // FP64 favours 4 points in the inverse passes, 8 in the forward ones; FP32 the mix below.
-template <typename TB> constexpr int E_XINV = sizeof(TB) == 8 ? 4 : 8;
-template <typename TB> constexpr int E_YINV = 4;
+#ifndef NS3D_EXINV32
+#define NS3D_EXINV32 8
+#endif
+#ifndef NS3D_EYINV32
+#define NS3D_EYINV32 4
+#endif
+template <typename TB> constexpr int E_XINV = sizeof(TB) == 8 ? 4 : NS3D_EXINV32;
+template <typename TB> constexpr int E_YINV = sizeof(TB) == 8 ? 4 : NS3D_EYINV32;
 template <typename TB> constexpr int E_C2C = 8;
 template <typename TB> constexpr int E_XFWD = sizeof(TB) == 8 ? 8 : 4;
 // one output field per block in the inverse passes (input re-read by 2 or 3 blocks)
@@ -58,8 +67,12 @@ template <typename TB> constexpr int E_XFWD = sizeof(TB) == 8 ? 8 : 4;
 #ifndef NS3D_OS_YINV
 #define NS3D_OS_YINV 0
 #endif
-// pencils per block in the z pass: ~128 threads
-template <int N> constexpr int PZ = (128 / ZPlan<N>::TPP) > 0 ? (128 / ZPlan<N>::TPP) : 1;
+// pencils per block in the z pass: ~128 threads (FP32: NS3D_PZT32)
+#ifndef NS3D_PZT32
+#define NS3D_PZT32 128
+#endif
+template <typename TB, int N>
+constexpr int PZ = ((sizeof(TB) == 8 ? 128 : NS3D_PZT32) / ZPlan<N>::TPP) > 0 ? ((sizeof(TB) == 8 ? 128 : NS3D_PZT32) / ZPlan<N>::TPP) : 1;
 
 #define HIPCHK(x)                                         \
     do {                                                  \
@@ -376,7 +389,7 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
     prof_begin(c, 2);
     st = with_size(c->Nz, [&](auto NZc) -> int {
         constexpr int NZ = decltype(NZc)::value;
-        constexpr int P = PZ<NZ>;
+        constexpr int P = PZ<TB, NZ>;
         const int nb = (int)nblk(c->npen, P);
         dim3 grid = xgrid(nb), blk(P * ZPlan<NZ>::TPP);
         const TB scale = (TB)(1.0 / (1.0 * c->Nx * c->Ny * c->Nz));
@@ -596,7 +609,7 @@ int c2r_3d(ns3d_ctx* c, const cplx<TB>* in, TB* out, cplx<TB>* tmp, bool scale) 
     if (st) return st;
     return with_size(c->Nz, [&](auto N) -> int {
         constexpr int NZ = decltype(N)::value;
-        constexpr int P = PZ<NZ>;
+        constexpr int P = PZ<TB, NZ>;
         const long npairs = c->npen / 2;
         const TB sc = scale ? (TB)(1.0 / (1.0 * c->Nx * c->Ny * c->Nz)) : (TB)1;
         const int nb = (int)nblk(npairs, P);
@@ -612,7 +625,7 @@ int r2c_3d(ns3d_ctx* c, const TB* in, cplx<TB>* out) {
     const long nc = c->nc;
     int st = with_size(c->Nz, [&](auto N) -> int {
         constexpr int NZ = decltype(N)::value;
-        constexpr int P = PZ<NZ>;
+        constexpr int P = PZ<TB, NZ>;
         const long npairs = c->npen / 2;
         const int nb = (int)nblk(npairs, P);
         hipLaunchKernelGGL((k_z_r2c<TB, NZ, P>), xgrid(nb), dim3(P * ZPlan<NZ>::TPP), 0, s, in, out, npairs,

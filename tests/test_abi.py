@@ -45,6 +45,16 @@ def test_bench_pass_bytes_total_159_fields():
     import bench
     nc = 256 * 256 * 129
     assert bench.step_bytes(nc, 16) == pytest.approx(159 * 16 * nc)
+    assert bench.step_bytes(nc, 8, 8) == pytest.approx(159 * 8 * nc)       # FP32 state and bulk
+    # FP32 bulk with the FP64 state: P1 reads 3 extra S32, P5 reads/writes 3+3(+3 in stages 2-3) extra
+    assert bench.step_bytes(nc, 8, 16) == pytest.approx((159 + 3 * 3 + 3 * 6 + 2 * 3) * 8 * nc)
+
+
+def test_profiled_traffic_lookup():
+    import bench
+    t = bench.profiled_traffic("fp64", "k_z_product")
+    assert t is not None and t > 0
+    assert bench.profiled_traffic("fp64", "no_such_kernel") is None
 
 
 def test_headers_compile_as_c_and_cpp(tmp_path):
