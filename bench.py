@@ -202,7 +202,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32", "fp32s"], default="fp64")
     ap.add_argument("--secondary", default="fp32,fp32s",
                     help="comma list of further precisions to time beside the primary (fp64, fp32, fp32s) or none")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--serve-requests", type=int, default=24,
