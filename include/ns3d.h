@@ -183,6 +183,22 @@ int ns3d_cfl_dt(ns3d_ctx *ctx, double cfl, const double *ux, const double *uy, c
 int ns3d_shapiro_error(ns3d_ctx *ctx, double t, const double *ux, const double *uy, const double *uz,
                        double *err_host);                                          /* Shapiro_test_case: L-inf error */
 
+/* ---- f1: FP64 vector kernels for the Krylov drivers (Arnoldi Gram-Schmidt with
+ *      DGKS re-orthogonalisation, implicit restarts, BiCGStab(L), Newton), the device
+ *      BLAS of Source/Arnoldi/cuda_supp.cu:180-990 (vector_dot_product_GPU,
+ *      matrixDotVector_part_GPU, matrixMultVector_part_GPU, matrixMultMatrix_GPU,
+ *      vectors_add_GPU).  A basis V is m device vectors of length n, vector j at
+ *      V + j*ldv; m, k <= 128.  Reductions are deterministic (fixed partition) and
+ *      return on the host after a stream synchronisation. ---- */
+int ns3d_vec_dot(ns3d_ctx *ctx, long long n, const double *x, const double *y, double *out_host);
+int ns3d_vec_gemv_t(ns3d_ctx *ctx, long long n, int m, const double *V, long long ldv, const double *w,
+                    double *h_host);                                    /* h = V^T w */
+int ns3d_vec_gemv(ns3d_ctx *ctx, long long n, int m, const double *V, long long ldv, const double *h_host,
+                  double alpha, double beta, double *y);                /* y = alpha V h + beta y */
+int ns3d_vec_gemm(ns3d_ctx *ctx, long long n, int m, int k, const double *V, long long ldv,
+                  const double *Q_host, double *W, long long ldw);      /* W = V Q, Q m x k column-major; W, V disjoint */
+int ns3d_vec_axpby(ns3d_ctx *ctx, long long n, double a, const double *x, double b, double *y);  /* y = a x + b y */
+
 /* ---- measurement: per-pass device time of the fused pipeline, recorded with
  *      HIP events on the context stream while enabled.  ms[5]/calls[5] are
  *      indexed 0 x-inverse, 1 y-inverse, 2 z product, 3 y-forward,

@@ -77,6 +77,11 @@ SIGNATURES = {
     "ns3d_jv_exponential": (None, [P, P, P]),
     "ns3d_reduce_minmax": (I, [P, LL, P, DP]),
     "ns3d_reduce_sum": (I, [P, LL, P, DP]),
+    "ns3d_vec_dot": (I, [P, LL, P, P, DP]),
+    "ns3d_vec_gemv_t": (I, [P, LL, I, P, LL, P, DP]),
+    "ns3d_vec_gemv": (I, [P, LL, I, P, LL, DP, D, D, P]),
+    "ns3d_vec_gemm": (I, [P, LL, I, I, P, LL, DP, P, LL]),
+    "ns3d_vec_axpby": (I, [P, LL, D, P, D, P]),
     "ns3d_kinetic_energy": (I, [P, P, P, P, DP]),
     "ns3d_dissipation": (I, [P, P, P, P, DP]),
     "ns3d_cfl_dt": (I, [P, D, P, P, P, DP]),

@@ -12,6 +12,7 @@
 
 #include "../../include/ns3d.h"
 #include "k_misc.hpp"
+#include "k_krylov.hpp"
 
 using namespace ns3d;
 
@@ -115,6 +116,12 @@ struct ns3d_ctx {
     int timesteps_period = 1;
     double tau = 1.0;
     zd* kry = nullptr;  // 3 fields for J.v unpacking
+    // Krylov vector kernels: reduction partials, results, pinned host copies, small dense matrix
+    double* kv_partial = nullptr;
+    double* kv_out = nullptr;
+    double* kv_host = nullptr;
+    double* kq = nullptr;
+    double* kq_host = nullptr;
     // a captured graph of nsteps RK3 steps for fixed state/forcing pointers, dt and Re
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -759,6 +766,17 @@ int rk3_scalar_one(ns3d_ctx* c, zd* const u[3], const zd* const f[3], double* th
     return st;
 }
 
+int ensure_kv(ns3d_ctx* c) {
+    if (c->kv_partial) return NS3D_OK;
+    if (hipMalloc((void**)&c->kv_partial, sizeof(double) * KV_COLS * KV_BLOCKS) != hipSuccess ||
+        hipMalloc((void**)&c->kv_out, sizeof(double) * KV_MAXM) != hipSuccess ||
+        hipMalloc((void**)&c->kq, sizeof(double) * KV_MAXM * KV_MAXM) != hipSuccess ||
+        hipHostMalloc((void**)&c->kv_host, sizeof(double) * KV_MAXM) != hipSuccess ||
+        hipHostMalloc((void**)&c->kq_host, sizeof(double) * KV_MAXM * KV_MAXM) != hipSuccess)
+        return NS3D_ERR_ALLOC;
+    return NS3D_OK;
+}
+
 }  // namespace
 
 // =====================================================================================
@@ -838,6 +856,11 @@ int ns3d_destroy(ns3d_ctx* c) {
     hipFree(c->red);
     hipFree(c->real_scratch);
     hipFree(c->kry);
+    hipFree(c->kv_partial);
+    hipFree(c->kv_out);
+    hipFree(c->kq);
+    if (c->kv_host) hipHostFree(c->kv_host);
+    if (c->kq_host) hipHostFree(c->kq_host);
     hipFree(c->uphys);
     hipFree(c->theta_stage);
     if (c->red_host) hipHostFree(c->red_host);
@@ -1363,6 +1386,64 @@ int ns3d_shapiro_error(ns3d_ctx* c, double t, const double* ux, const double* uy
     int st = reduce<1, SRC_SHAPIRO>(c, r, c->nr, mm);
     if (!st) *err = mm[1];
     return st;
+}
+
+// ---- f1: FP64 vector kernels of the Krylov drivers (Source/Arnoldi/cuda_supp.cu)
+int ns3d_vec_gemv_t(ns3d_ctx* c, long long n, int m, const double* V, long long ldv, const double* w, double* h) {
+    if (!c || !V || !w || !h || n < 1 || m < 1 || m > KV_MAXM || ldv < n) return NS3D_ERR_ARG;
+    int st = ensure_kv(c);
+    if (st) return st;
+    const int nb = (int)std::min<long long>(KV_BLOCKS, (n + KV_THREADS - 1) / KV_THREADS);
+    for (int c0 = 0; c0 < m; c0 += KV_COLS) {
+        const int nc = std::min(KV_COLS, m - c0);
+        hipLaunchKernelGGL(k_kv_gemv_t_partial, dim3(nb), dim3(KV_THREADS), 0, c->stream, (long)n, nc, V, (long)ldv, c0, w,
+                           c->kv_partial);
+        hipLaunchKernelGGL(k_kv_final, dim3(nc), dim3(KV_THREADS), 0, c->stream, c->kv_partial, nb, c->kv_out + c0);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->kv_host, c->kv_out, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < m; j++) h[j] = c->kv_host[j];
+    return NS3D_OK;
+}
+
+int ns3d_vec_dot(ns3d_ctx* c, long long n, const double* x, const double* y, double* out) {
+    return ns3d_vec_gemv_t(c, n, 1, x, n, y, out);
+}
+
+int ns3d_vec_gemv(ns3d_ctx* c, long long n, int m, const double* V, long long ldv, const double* h, double alpha,
+                  double beta, double* y) {
+    if (!c || !V || !h || !y || n < 1 || m < 1 || m > KV_MAXM || ldv < n) return NS3D_ERR_ARG;
+    KvCoef hc;
+    for (int j = 0; j < KV_MAXM; j++) hc.h[j] = j < m ? h[j] : 0.0;
+    hipLaunchKernelGGL(k_kv_gemv, dim3(nblk(n, KV_THREADS)), dim3(KV_THREADS), 0, c->stream, (long)n, m, V, (long)ldv, hc,
+                       alpha, beta, y);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+
+int ns3d_vec_gemm(ns3d_ctx* c, long long n, int m, int k, const double* V, long long ldv, const double* Q, double* W,
+                  long long ldw) {
+    if (!c || !V || !Q || !W || n < 1 || m < 1 || k < 1 || m > KV_MAXM || k > KV_MAXM || ldv < n || ldw < n)
+        return NS3D_ERR_ARG;
+    if (W < V + (long long)(m - 1) * ldv + n && V < W + (long long)(k - 1) * ldw + n) return NS3D_ERR_ARG;  // overlap
+    int st = ensure_kv(c);
+    if (st) return st;
+    for (long j = 0; j < (long)m * k; j++) c->kq_host[j] = Q[j];
+    HIPCHK(hipMemcpyAsync(c->kq, c->kq_host, sizeof(double) * m * k, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_kv_gemm, dim3(nblk(n, KV_THREADS), k), dim3(KV_THREADS), 0, c->stream, (long)n, m, k, V, (long)ldv,
+                       (const double*)c->kq, W, (long)ldw);
+    HIPCHK(hipGetLastError());
+    // kq_host is reused by the next call: wait for the copy (and the product)
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS3D_OK;
+}
+
+int ns3d_vec_axpby(ns3d_ctx* c, long long n, double a, const double* x, double b, double* y) {
+    if (!c || !x || !y || n < 1) return NS3D_ERR_ARG;
+    hipLaunchKernelGGL(k_kv_axpby, dim3(nblk(n, KV_THREADS)), dim3(KV_THREADS), 0, c->stream, (long)n, a, x, b, y);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
 }
 
 }  // extern "C"

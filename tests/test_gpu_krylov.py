@@ -1,0 +1,99 @@
+"""Krylov stack on the GPU: the FP64 vector kernels against numpy, the
+implicitly restarted Arnoldi on the Navier-Stokes J.v against the dense
+Jacobian assembled from the oracle, and Newton-Krylov to the reference's
+laminar Kolmogorov steady state (FP64, and with an FP32 Jacobian)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _odd(u):
+    # purely imaginary coefficients: the odd-symmetric subspace the reduced vectors describe
+    return [1j * c.imag for c in u]
+
+
+def test_vector_kernels_match_numpy_and_repeat():
+    import torch
+    from ns3dperiodic_amd import NS3D, krylov
+    print("\n[case] krylov vector kernels", flush=True)
+    g = NS3D(64)
+    ops, ref = krylov.GpuOps(g), krylov.NumpyOps()
+    n = g.vector_length_reduced()
+    m, k = 30, 7
+    rng = np.random.default_rng(0)
+    Vh = rng.standard_normal((m, n))
+    wh = rng.standard_normal(n)
+    V, w = ops.from_host(Vh), ops.from_host(wh)
+    h = ops.gemv_t(V, m, w)
+    assert np.allclose(h, ref.gemv_t(Vh, m, wh), rtol=1e-12, atol=1e-12 * np.sqrt(n))
+    assert np.array_equal(h, ops.gemv_t(V, m, w))                  # deterministic
+    assert abs(ops.dot(w, w) - float(wh @ wh)) < 1e-12 * float(wh @ wh)
+    y = ops.from_host(wh)
+    ops.gemv(V, m, h, -0.5, 2.0, y)
+    assert np.allclose(ops.to_host(y), 2.0 * wh - 0.5 * h @ Vh, rtol=1e-12, atol=1e-9)
+    Q = rng.standard_normal((m, k))
+    W = ops.zeros(n, k)
+    ops.gemm(V, m, Q, W)
+    assert np.allclose(ops.to_host(W), Q.T @ Vh, rtol=1e-12, atol=1e-10)
+    x = ops.from_host(Vh[0])
+    ops.axpby(3.0, x, -1.0, y)
+    assert np.allclose(ops.to_host(y), 3.0 * Vh[0] - (2.0 * wh - 0.5 * h @ Vh), atol=1e-9)
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("which", ["LR", "LM"])
+def test_eigs_ns_jacobian_against_dense_oracle(which):
+    import torch
+    from ns3dperiodic_amd import NS3D, fields, krylov
+    from oracle.oracle import Oracle
+    print(f"\n[case] IRA on NS J.v at 8^3 ({which})", flush=True)
+    N, Re = 8, 40.0
+    g = NS3D(N, Re=Re, dt=1e-3)
+    o = Oracle(N, Re=Re, dt=1e-3)
+    u = _odd(fields.random_solenoidal(N, seed=7, energy=2.0))
+    n = g.vector_length_reduced()
+    # dense Jacobian, one oracle J.v per unit vector
+    J = np.empty((n, n))
+    e = np.zeros(n)
+    for i in range(n):
+        e[i] = 1.0
+        J[:, i] = o.jv_reduced(u, e)
+        e[i] = 0.0
+    ref = np.linalg.eigvals(J)
+    ref = ref[krylov._order(ref, which)][:6]
+    ops = krylov.NSOperators(g, [torch.from_numpy(c).cuda() for c in u])
+    out = krylov.eigs(ops.ops, ops.jv, n, k=6, m=30, which=which, tol=1e-11, max_iter=500)
+    print(f"[krylov] {which}: iterations {out['iterations']}, values {np.round(out['values'], 6)}", flush=True)
+    for lam in ref:
+        assert np.min(np.abs(out["values"] - lam)) < 1e-8 * max(1.0, abs(lam)), (lam, out["values"])
+    assert np.all(out["residuals"] < 1e-7 * max(1.0, np.abs(ref).max()))
+
+
+@pytest.mark.parametrize("jac_precision", ["fp64", "fp32"])
+def test_newton_to_laminar_kolmogorov(jac_precision):
+    import torch
+    from ns3dperiodic_amd import NS3D, fields, krylov
+    print(f"\n[case] Newton-Krylov, Jacobian in {jac_precision}", flush=True)
+    N, Re = 16, 10.0
+    ust, f = fields.reference_kolmogorov(N, n=1, Re=Re)
+    g64 = NS3D(N, Re=Re, dt=1e-3)
+    fd = [torch.from_numpy(c).cuda() for c in f]
+    rhs_ops = krylov.NSOperators(g64, f=fd)
+    gj = g64 if jac_precision == "fp64" else NS3D(N, Re=Re, dt=1e-3, precision="fp32")
+    jac_ops = krylov.NSOperators(gj, f=fd)
+    ops = rhs_ops.ops
+    xs = ops.to_host(g64.pack_reduced([torch.from_numpy(c).cuda() for c in ust]))
+    F = ops.zeros(len(xs))
+    rhs_ops.rhs(ops.from_host(xs), F)
+    assert np.linalg.norm(ops.to_host(F)) < 1e-9 * np.linalg.norm(xs)    # the reference state is steady
+    rng = np.random.default_rng(3)
+    pert = ops.to_host(g64.pack_reduced([torch.from_numpy(c).cuda() for c in _odd(fields.random_solenoidal(N, seed=4))]))
+    x = ops.from_host(xs + 0.05 * np.linalg.norm(xs) / np.linalg.norm(pert) * pert)
+    out = krylov.newton(ops, rhs_ops.rhs, jac_ops.jacobian, x, tol=1e-9 * np.linalg.norm(xs), max_iter=60,
+                        lin_tol=1e-6 if jac_precision == "fp32" else 1e-10)
+    print(f"[krylov] Newton {jac_precision}: {out['iterations']} steps, |F| history {['%.1e' % v for v in out['history']]}",
+          flush=True)
+    assert out["converged"]
+    err = np.linalg.norm(ops.to_host(x) - xs) / np.linalg.norm(xs)
+    assert err < 1e-9

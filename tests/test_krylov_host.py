@@ -1,0 +1,102 @@
+"""Krylov drivers (Arnoldi + DGKS, implicit restarts, BiCGStab(L), damped
+Newton) on dense host problems, where numpy's direct solvers pin the answers."""
+import numpy as np
+import pytest
+
+from ns3dperiodic_amd import krylov
+
+OPS = krylov.NumpyOps()
+
+
+def _mv(A):
+    def mv(x, y):
+        y[...] = A @ x
+        return y
+    return mv
+
+
+def test_arnoldi_factorisation():
+    rng = np.random.default_rng(1)
+    n, m = 120, 20
+    A = rng.standard_normal((n, n))
+    V = OPS.zeros(n, m)
+    H = np.zeros((m, m))
+    f = rng.standard_normal(n)
+    assert krylov.arnoldi_extend(OPS, _mv(A), V, H, f, 0, m) == 0
+    assert np.allclose(V @ V.T, np.eye(m), atol=1e-13)
+    em = np.zeros(m)
+    em[-1] = 1
+    assert np.allclose(A @ V.T, V.T @ H + np.outer(f, em), atol=1e-11)
+    assert np.allclose(np.tril(H, -2), 0)
+
+
+@pytest.mark.parametrize("which", ["LM", "LR"])
+def test_eigs_matches_dense(which):
+    rng = np.random.default_rng(2)
+    n = 300
+    # nonsymmetric, with a few well separated eigenvalues (real and complex) on top of a bulk
+    A = rng.standard_normal((n, n)) / np.sqrt(n)
+    Qm, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    D = np.zeros((n, n))
+    D[:2, :2] = [[3.0, 1.5], [-1.5, 3.0]]      # 3 +- 1.5i
+    D[2, 2], D[3, 3], D[4, 4], D[5, 5] = 2.6, -2.8, 2.2, 1.9
+    A = A + Qm @ D @ Qm.T
+    out = krylov.eigs(OPS, _mv(A), n, k=6, m=30, which=which, tol=1e-12)
+    ref = np.linalg.eigvals(A)
+    ref = ref[krylov._order(ref, which)][:6]
+    got = out["values"]
+    for lam in ref:
+        assert np.min(np.abs(got - lam)) < 1e-9 * max(1, abs(lam)), (lam, got)
+    assert np.all(out["residuals"] < 1e-8)
+
+
+@pytest.mark.parametrize("L", [1, 2, 4, 6])
+def test_bicgstab_l_solves(L):
+    rng = np.random.default_rng(3)
+    n = 200
+    A = 3.0 * np.eye(n) + rng.standard_normal((n, n)) / np.sqrt(n)
+    b = rng.standard_normal(n)
+    x = np.zeros(n)
+    flag, it, res = krylov.bicgstab_l(OPS, _mv(A), b, x, L=L, tol=1e-12)
+    assert flag == 0 and res < 1e-11
+    assert np.allclose(x, np.linalg.solve(A, b), atol=1e-10)
+
+
+def test_newton_nonlinear_system():
+    rng = np.random.default_rng(4)
+    n = 60
+    A = 4.0 * np.eye(n) + rng.standard_normal((n, n)) / np.sqrt(n)
+    xs = rng.standard_normal(n)
+    b = A @ xs + 0.1 * xs ** 3          # F(x) = A x + 0.1 x^3 - b, root xs
+
+    def rhs(x, out):
+        out[...] = A @ x + 0.1 * x ** 3 - b
+        return out
+
+    def jac(x):
+        d = 0.3 * x ** 2
+        x0 = x.copy()
+
+        def mv(v, y):
+            y[...] = A @ v + 0.3 * x0 ** 2 * v
+            return y
+        return mv
+
+    x = np.zeros(n)
+    out = krylov.newton(OPS, rhs, jac, x, tol=1e-11, max_iter=100)
+    assert out["converged"] and out["iterations"] >= 3
+    assert np.allclose(x, xs, atol=1e-10)
+    # damping 0.8: the residual falls at least geometrically
+    h = out["history"]
+    assert h[-1] < 1e-11 and h[1] < h[0]
+
+
+def test_shift_application_keeps_hessenberg_similarity():
+    rng = np.random.default_rng(5)
+    m = 12
+    H = np.triu(rng.standard_normal((m, m)), -1)
+    vals = np.linalg.eigvals(H)
+    H2, Q = krylov._apply_shifts(H.copy(), vals[:4])
+    assert np.allclose(Q.T @ Q, np.eye(m), atol=1e-12)
+    assert np.allclose(Q.T @ H @ Q, H2, atol=1e-10)
+    assert np.allclose(np.sort_complex(np.linalg.eigvals(H2)), np.sort_complex(vals), atol=1e-9)
