@@ -199,6 +199,15 @@ int ns3d_vec_gemm(ns3d_ctx *ctx, long long n, int m, int k, const double *V, lon
                   const double *Q_host, double *W, long long ldw);      /* W = V Q, Q m x k column-major; W, V disjoint */
 int ns3d_vec_axpby(ns3d_ctx *ctx, long long n, double a, const double *x, double b, double *y);  /* y = a x + b y */
 
+/* ---- f4: diagnostics of the reference driver (Source/math_support.cu:712-987) ---- */
+int ns3d_curl(ns3d_ctx *ctx, const double *ux, const double *uy, const double *uz,
+              double *rot_x, double *rot_y, double *rot_z, int fix);    /* get_curl; fix=0 keeps the
+                                                                          reference's uz(j+1)-ux(j-1) in rot_y */
+int ns3d_helmholtz_filter(ns3d_ctx *ctx, double eps, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
+                          ns3d_cplx *fx, ns3d_cplx *fy, ns3d_cplx *fz);   /* Helmholz_Fourier_Filter (spectral part) */
+int ns3d_cutoff_filter(ns3d_ctx *ctx, double radius, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
+                       ns3d_cplx *fx, ns3d_cplx *fy, ns3d_cplx *fz);      /* CutOff_Fourier_Filter (spectral part) */
+
 /* ---- measurement: per-pass device time of the fused pipeline, recorded with
  *      HIP events on the context stream while enabled.  ms[5]/calls[5] are
  *      indexed 0 x-inverse, 1 y-inverse, 2 z product, 3 y-forward,

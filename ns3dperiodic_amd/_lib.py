@@ -82,6 +82,9 @@ SIGNATURES = {
     "ns3d_vec_gemv": (I, [P, LL, I, P, LL, DP, D, D, P]),
     "ns3d_vec_gemm": (I, [P, LL, I, I, P, LL, DP, P, LL]),
     "ns3d_vec_axpby": (I, [P, LL, D, P, D, P]),
+    "ns3d_curl": (I, [P] + [P] * 6 + [I]),
+    "ns3d_helmholtz_filter": (I, [P, D] + [P] * 6),
+    "ns3d_cutoff_filter": (I, [P, D] + [P] * 6),
     "ns3d_kinetic_energy": (I, [P, P, P, P, DP]),
     "ns3d_dissipation": (I, [P, P, P, P, DP]),
     "ns3d_cfl_dt": (I, [P, D, P, P, P, DP]),

@@ -13,6 +13,7 @@
 #include "../../include/ns3d.h"
 #include "k_misc.hpp"
 #include "k_krylov.hpp"
+#include "k_diag.hpp"
 
 using namespace ns3d;
 
@@ -1444,6 +1445,42 @@ int ns3d_vec_axpby(ns3d_ctx* c, long long n, double a, const double* x, double b
     hipLaunchKernelGGL(k_kv_axpby, dim3(nblk(n, KV_THREADS)), dim3(KV_THREADS), 0, c->stream, (long)n, a, x, b, y);
     HIPCHK(hipGetLastError());
     return NS3D_OK;
+}
+
+// ---- f4: diagnostics of the reference driver (Source/math_support.cu:712-987)
+int ns3d_curl(ns3d_ctx* c, const double* ux, const double* uy, const double* uz, double* rx, double* ry, double* rz,
+              int fix) {
+    if (!c || !ux || !uy || !uz || !rx || !ry || !rz) return NS3D_ERR_ARG;
+    Grid3 G{c->Nx, c->Ny, c->Nz};
+    hipLaunchKernelGGL(k_curl, dim3(nblk(c->nr, 256)), dim3(256), 0, c->stream, ux, uy, uz, rx, ry, rz, G, c->Lx / c->Nx,
+                       c->Ly / c->Ny, c->Lz / c->Nz, fix ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+
+static int spectral_filter(ns3d_ctx* c, int kind, double p, const ns3d_cplx* ix, const ns3d_cplx* iy, const ns3d_cplx* iz,
+                           ns3d_cplx* ox, ns3d_cplx* oy, ns3d_cplx* oz) {
+    if (!c || !ix || !iy || !iz || !ox || !oy || !oz) return NS3D_ERR_ARG;
+    PtrArr<const zd, 3> in{{(const zd*)ix, (const zd*)iy, (const zd*)iz}};
+    PtrArr<zd, 3> out{{(zd*)ox, (zd*)oy, (zd*)oz}};
+    if (kind == 0)
+        hipLaunchKernelGGL(k_spectral_filter<0>, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, in, out, c->nc, c->Nx,
+                           c->Ny, c->Mz, c->Lx, c->Ly, c->Lz, p);
+    else
+        hipLaunchKernelGGL(k_spectral_filter<1>, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, in, out, c->nc, c->Nx,
+                           c->Ny, c->Mz, c->Lx, c->Ly, c->Lz, p);
+    HIPCHK(hipGetLastError());
+    return NS3D_OK;
+}
+
+int ns3d_helmholtz_filter(ns3d_ctx* c, double eps, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz,
+                          ns3d_cplx* fx, ns3d_cplx* fy, ns3d_cplx* fz) {
+    return spectral_filter(c, 0, eps, ux, uy, uz, fx, fy, fz);
+}
+
+int ns3d_cutoff_filter(ns3d_ctx* c, double radius, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz,
+                       ns3d_cplx* fx, ns3d_cplx* fy, ns3d_cplx* fz) {
+    return spectral_filter(c, 1, radius, ux, uy, uz, fx, fy, fz);
 }
 
 }  // extern "C"

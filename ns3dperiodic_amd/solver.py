@@ -266,6 +266,23 @@ class NS3D:
         check(self.lib.ns3d_last_error(self.ctx), "ns3d_jv_exponential")
         return vec_out
 
+    # ------------------------------------------------------------ f4 diagnostics
+    def get_curl(self, u, fix=False):
+        """get_curl: vorticity of physical u by central differences (fix: see ns3d_curl)."""
+        r = self.rfield(3)
+        check(self.lib.ns3d_curl(self.ctx, *self._r3(u), *self._r3(r), int(fix)), "ns3d_curl")
+        return r
+
+    def Helmholz_Fourier_Filter(self, u_hat, eps):
+        out = self.cfield(3)
+        check(self.lib.ns3d_helmholtz_filter(self.ctx, eps, *self._c3(u_hat), *self._c3(out)), "ns3d_helmholtz_filter")
+        return out
+
+    def CutOff_Fourier_Filter(self, u_hat, radius):
+        out = self.cfield(3)
+        check(self.lib.ns3d_cutoff_filter(self.ctx, radius, *self._c3(u_hat), *self._c3(out)), "ns3d_cutoff_filter")
+        return out
+
     # ------------------------------------------------------------ a11 / a12 (host results)
     def compute_reduction(self, x):
         out = (ctypes.c_double * 2)()

@@ -303,3 +303,22 @@ def test_physical_mode(shape):
     ud = to_dev(u)
     g.RK3_SSP(ud, to_dev(f), steps=3)
     assert rel_l2(to_host(ud), o.rk3(u, f, steps=3)) < 3 * TOL
+
+
+@pytest.mark.parametrize("fix", [False, True])
+def test_diagnostics_curl_and_filters(pair, fix):
+    """f4 diagnostics: get_curl (both y-component variants), Helmholtz and cut-off filters."""
+    from oracle import diagnostics as od
+    from tests.gpu_util import to_dev, to_host
+    g, o = pair
+    sh = (o.Nx, o.Ny, o.Nz)
+    u_hat = _field(sh, seed=51)
+    u = [o.fft_c2r(c) for c in u_hat]
+    r = to_host(g.get_curl(to_dev(u), fix=fix))
+    ro = od.curl(*u, 2 * np.pi / o.Nx, 2 * np.pi / o.Ny, 2 * np.pi / o.Nz, fix=fix)
+    assert rel_l2(r, list(ro)) < 1e-14
+    fh = to_host(g.Helmholz_Fourier_Filter(to_dev(u_hat), 0.2))
+    assert rel_l2(fh, od.helmholtz_filter(u_hat, 0.2, 2 * np.pi, 2 * np.pi, 2 * np.pi)) < 1e-15
+    fc = to_host(g.CutOff_Fourier_Filter(to_dev(u_hat), 0.1))
+    for a, b in zip(fc, od.cutoff_filter(u_hat, 0.1)):
+        np.testing.assert_array_equal(a, b)
