@@ -208,6 +208,12 @@ int ns3d_helmholtz_filter(ns3d_ctx *ctx, double eps, const ns3d_cplx *ux, const 
 int ns3d_cutoff_filter(ns3d_ctx *ctx, double radius, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
                        ns3d_cplx *fx, ns3d_cplx *fy, ns3d_cplx *fz);      /* CutOff_Fourier_Filter (spectral part) */
 
+/* ---- f3: return map.  Velocity at grid point (j,k,l) evaluated from the half spectrum in one
+ *      pass (return_vector3_solution, Source/return_map.cu:110-160, which inverse-transforms
+ *      all three fields to read one point); out3_host = (ux, uy, uz). ---- */
+int ns3d_probe(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz, int j, int k, int l,
+               double *out3_host);
+
 /* ---- measurement: per-pass device time of the fused pipeline, recorded with
  *      HIP events on the context stream while enabled.  ms[5]/calls[5] are
  *      indexed 0 x-inverse, 1 y-inverse, 2 z product, 3 y-forward,

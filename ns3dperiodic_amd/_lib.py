@@ -85,6 +85,7 @@ SIGNATURES = {
     "ns3d_curl": (I, [P] + [P] * 6 + [I]),
     "ns3d_helmholtz_filter": (I, [P, D] + [P] * 6),
     "ns3d_cutoff_filter": (I, [P, D] + [P] * 6),
+    "ns3d_probe": (I, [P, P, P, P, I, I, I, DP]),
     "ns3d_kinetic_energy": (I, [P, P, P, P, DP]),
     "ns3d_dissipation": (I, [P, P, P, P, DP]),
     "ns3d_cfl_dt": (I, [P, D, P, P, P, DP]),

@@ -2,6 +2,7 @@
 // vorticity by central differences and the two spectral LES filters.
 #pragma once
 #include "k_misc.hpp"
+#include "k_krylov.hpp"
 
 namespace ns3d {
 
@@ -48,6 +49,43 @@ k_spectral_filter(PtrArr<const cplx<double>, 3> in, PtrArr<cplx<double>, 3> out,
     for (int b = 0; b < 3; b++) {
         const cplx<double> v = in.p[b][g];
         out.p[b][g] = mk<double>(f * v.x, f * v.y);
+    }
+}
+
+}  // namespace ns3d
+
+namespace ns3d {
+
+// Physical velocity at grid point (jp, kp, lp) straight from the half
+// spectrum: u = (1/(NxNyNz)) sum w_l Re(u^_{jkl} exp(2 pi i (j jp/Nx + k kp/Ny + l lp/Nz))),
+// w_l = 1 on the l = 0 and l = Nz/2 planes and 2 elsewhere (the C2R convention of
+// the library's transforms).  One pass over the three fields replaces three
+// inverse 3D transforms (return_vector3_solution, Source/return_map.cu:110-160).
+// partial[c*gridDim.x + b], c < 3.
+__global__ void __launch_bounds__(KV_THREADS)
+k_probe_partial(PtrArr<const cplx<double>, 3> u, long nc, int Nx, int Ny, int Nz, int Mz, int jp, int kp, int lp,
+                double* __restrict__ partial) {
+    __shared__ double sh[KV_THREADS / 64];
+    double acc[3] = {0.0, 0.0, 0.0};
+    const long stride = (long)gridDim.x * KV_THREADS;
+    for (long g = (long)blockIdx.x * KV_THREADS + threadIdx.x; g < nc; g += stride) {
+        const int l = (int)(g % Mz);
+        const long t = g / Mz;
+        const int k = (int)(t % Ny), j = (int)(t / Ny);
+        const long a = ((long)j * jp) % Nx, b = ((long)k * kp) % Ny, c = ((long)l * lp) % Nz;
+        double s, co;
+        sincospi(2.0 * ((double)a / Nx + (double)b / Ny + (double)c / Nz), &s, &co);
+        const double w = (l == 0 || 2 * l == Nz) ? 1.0 : 2.0;
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            const cplx<double> v = u.p[f][g];
+            acc[f] += w * (v.x * co - v.y * s);
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        const double r = kv_block_sum(acc[f], sh);
+        if (threadIdx.x == 0) partial[(long)f * gridDim.x + blockIdx.x] = r;
     }
 }
 

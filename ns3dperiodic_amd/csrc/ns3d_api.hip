@@ -1483,4 +1483,24 @@ int ns3d_cutoff_filter(ns3d_ctx* c, double radius, const ns3d_cplx* ux, const ns
     return spectral_filter(c, 1, radius, ux, uy, uz, fx, fy, fz);
 }
 
+// ---- f3: the return map's probe (return_vector3_solution, Source/return_map.cu:110-160)
+int ns3d_probe(ns3d_ctx* c, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz, int j, int k, int l,
+               double* out3) {
+    if (!c || !ux || !uy || !uz || !out3 || j < 0 || k < 0 || l < 0 || j >= c->Nx || k >= c->Ny || l >= c->Nz)
+        return NS3D_ERR_ARG;
+    int st = ensure_kv(c);
+    if (st) return st;
+    PtrArr<const zd, 3> u{{(const zd*)ux, (const zd*)uy, (const zd*)uz}};
+    const int nb = (int)std::min<long>(KV_BLOCKS, (c->nc + KV_THREADS - 1) / KV_THREADS);
+    hipLaunchKernelGGL(k_probe_partial, dim3(nb), dim3(KV_THREADS), 0, c->stream, u, c->nc, c->Nx, c->Ny, c->Nz, c->Mz, j,
+                       k, l, c->kv_partial);
+    hipLaunchKernelGGL(k_kv_final, dim3(3), dim3(KV_THREADS), 0, c->stream, c->kv_partial, nb, c->kv_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->kv_host, c->kv_out, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double inv = 1.0 / (1.0 * c->Nx * c->Ny * c->Nz);
+    for (int b = 0; b < 3; b++) out3[b] = c->kv_host[b] * inv;
+    return NS3D_OK;
+}
+
 }  // extern "C"

@@ -12,6 +12,7 @@ hand-written gfx950 kernels of libns3d_hip.so.
 import ctypes
 import math
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -282,6 +283,13 @@ class NS3D:
         out = self.cfield(3)
         check(self.lib.ns3d_cutoff_filter(self.ctx, radius, *self._c3(u_hat), *self._c3(out)), "ns3d_cutoff_filter")
         return out
+
+    # ------------------------------------------------------------ f3 return map probe
+    def probe(self, u_hat, j, k, l):
+        """(ux, uy, uz) at grid point (j, k, l) of the spectral field u_hat (host floats)."""
+        out = (ctypes.c_double * 3)()
+        check(self.lib.ns3d_probe(self.ctx, *self._c3(u_hat), j, k, l, out), "ns3d_probe")
+        return np.array(out[:])
 
     # ------------------------------------------------------------ a11 / a12 (host results)
     def compute_reduction(self, x):
