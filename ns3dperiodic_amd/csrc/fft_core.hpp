@@ -14,6 +14,7 @@
 // y[jj + r*N/Rlast]: callers load and store those patterns directly from and
 // to global memory, and only the inner exchanges touch LDS.
 #pragma once
+#include <type_traits>
 #include "ns3d_common.hpp"
 
 namespace ns3d {
@@ -168,6 +169,16 @@ template <> struct ZPlan<192> { using R = Seq<3, 4, 4, 4>; static constexpr int 
 template <> struct ZPlan<256> { using R = Seq<4, 4, 4, 4>; static constexpr int TPP = 64; };
 template <> struct ZPlan<384> { using R = Seq<3, 2, 4, 4, 4>; static constexpr int TPP = 96; };
 template <> struct ZPlan<512> { using R = Seq<2, 4, 4, 4, 4>; static constexpr int TPP = 128; };
+
+// z-pass plan by bulk precision: ZPlan (4 points per thread), or for FP32
+// with NS3D_ZPLAN32 = 8 the 8-point Plan (one LDS exchange fewer per transform).
+#ifndef NS3D_ZPLAN32
+#define NS3D_ZPLAN32 8
+#endif
+template <typename TB, int N> struct ZSel {
+    using type = typename std::conditional<(sizeof(TB) == 4 && NS3D_ZPLAN32 == 8), Plan<N>, ZPlan<N>>::type;
+};
+template <typename TB, int N> using ZPlanT = typename ZSel<TB, N>::type;
 
 // Registers per thread for a plan: max over passes of ceil((N/R)/TPP)*R.
 template <int N, int TPP, typename S> struct PlanRegs;

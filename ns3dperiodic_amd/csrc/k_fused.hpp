@@ -230,19 +230,20 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
 // C2R convention: x,y passes already done; only Re of l=0 and l=Nz/2 is used.
 template <typename TB, int NZ, int P, bool TRUNC = false>
 struct ZPencil {
-    static constexpr int TPP = ZPlan<NZ>::TPP;
+    static constexpr int TPP = ZPlanT<TB, NZ>::TPP;
     static constexpr int PAD = (sizeof(TB) == 8) ? 1 : 2;
 #ifndef NS3D_ZSWZ
 #define NS3D_ZSWZ 16
 #endif
 #ifndef NS3D_ZSWZ32
-#define NS3D_ZSWZ32 0
+#define NS3D_ZSWZ32 16
 #endif
-    // LDS swizzle group in elements (0: none); measured: it pays for FP64 only
+    // LDS swizzle group in elements (0: none); measured gains: FP64 with the 4-point
+    // plan, FP32 with the 8-point plan
     static constexpr int SWG = sizeof(TB) == 8 ? NS3D_ZSWZ : NS3D_ZSWZ32;
     using LA = typename std::conditional<SWG == 0, PencilMajor<NZ, PAD>, PencilMajorSw<NZ, PAD, (SWG == 0 ? 1 : SWG)>>::type;
-    using FI = PencilFFT<TB, NZ, TPP, +1, typename ZPlan<NZ>::R, LA>;
-    using FF = PencilFFT<TB, NZ, TPP, -1, typename SeqRev<typename ZPlan<NZ>::R>::type, LA>;
+    using FI = PencilFFT<TB, NZ, TPP, +1, typename ZPlanT<TB, NZ>::R, LA>;
+    using FF = PencilFFT<TB, NZ, TPP, -1, typename SeqRev<typename ZPlanT<TB, NZ>::R>::type, LA>;
     static_assert(FI::RL == FF::R0 && FI::BL == FF::B0, "mirror plans must share the product layout");
     static constexpr int E = FI::E > FF::E ? FI::E : FF::E;
     static constexpr int NPT = FI::BL * FI::RL;  // physical points per thread
@@ -385,7 +386,7 @@ struct ZPencil {
 };
 
 template <typename TB, int NZ, int P, int NF>
-__global__ void __attribute__((amdgpu_waves_per_eu(NF == 1 ? NS3D_WAVES_Z : 2, 8))) __launch_bounds__(P* ZPlan<NZ>::TPP)
+__global__ void __attribute__((amdgpu_waves_per_eu(NF == 1 ? NS3D_WAVES_Z : 2, 8))) __launch_bounds__((P * ZPlanT<TB, NZ>::TPP))
 k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, long npen, int Mz, int nb,
             SpecTables tab, const cplx<TB>* __restrict__ twg, TB scale, TB* __restrict__ uphys, long nr) {
     // B is zero on z-planes l > lmax (never written by the y pass): TRUNC skips them

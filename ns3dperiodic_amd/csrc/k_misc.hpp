@@ -7,7 +7,7 @@ namespace ns3d {
 // ------------------------------------------------------- z C2R / R2C (single fields)
 // Two neighbouring (x,y) pencils share one complex FFT: Y = X_p0 + i X_p1.
 template <typename TB, int NZ, int P>
-__global__ void __launch_bounds__(P* ZPlan<NZ>::TPP)
+__global__ void __launch_bounds__((P * ZPlanT<TB, NZ>::TPP))
 k_z_c2r(const cplx<TB>* __restrict__ in, TB* __restrict__ out, long npairs, int Mz, int nb, SpecTables tab,
         const cplx<TB>* __restrict__ twg, TB scale) {
     using Z = ZPencil<TB, NZ, P>;
@@ -36,7 +36,7 @@ k_z_c2r(const cplx<TB>* __restrict__ in, TB* __restrict__ out, long npairs, int 
 }
 
 template <typename TB, int NZ, int P>
-__global__ void __launch_bounds__(P* ZPlan<NZ>::TPP)
+__global__ void __launch_bounds__((P * ZPlanT<TB, NZ>::TPP))
 k_z_r2c(const TB* __restrict__ in, cplx<TB>* __restrict__ out, long npairs, int Mz, int nb,
         const cplx<TB>* __restrict__ twg) {
     using Z = ZPencil<TB, NZ, P>;

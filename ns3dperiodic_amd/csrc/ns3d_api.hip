@@ -73,8 +73,13 @@ template <typename TB> constexpr int E_XFWD = sizeof(TB) == 8 ? 8 : 4;
 #ifndef NS3D_PZT32
 #define NS3D_PZT32 128
 #endif
+// (measured: smaller blocks beat whole-wavefront ones, e.g. 96 vs 192 threads at 192^3)
+#ifndef NS3D_PZT64
+#define NS3D_PZT64 128
+#endif
+template <typename TB> constexpr int PZT = sizeof(TB) == 8 ? NS3D_PZT64 : NS3D_PZT32;
 template <typename TB, int N>
-constexpr int PZ = ((sizeof(TB) == 8 ? 128 : NS3D_PZT32) / ZPlan<N>::TPP) > 0 ? ((sizeof(TB) == 8 ? 128 : NS3D_PZT32) / ZPlan<N>::TPP) : 1;
+constexpr int PZ = (PZT<TB> / ZPlanT<TB, N>::TPP) > 0 ? (PZT<TB> / ZPlanT<TB, N>::TPP) : 1;
 
 #define HIPCHK(x)                                         \
     do {                                                  \
@@ -399,7 +404,7 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
         constexpr int NZ = decltype(NZc)::value;
         constexpr int P = PZ<TB, NZ>;
         const int nb = (int)nblk(c->npen, P);
-        dim3 grid = xgrid(nb), blk(P * ZPlan<NZ>::TPP);
+        dim3 grid = xgrid(nb), blk(P * ZPlanT<TB, NZ>::TPP);
         const TB scale = (TB)(1.0 / (1.0 * c->Nx * c->Ny * c->Nz));
         TB* up = nf == 1 ? (TB*)c->uphys_out : nullptr;
         if (nf == 1)
@@ -621,7 +626,7 @@ int c2r_3d(ns3d_ctx* c, const cplx<TB>* in, TB* out, cplx<TB>* tmp, bool scale) 
         const long npairs = c->npen / 2;
         const TB sc = scale ? (TB)(1.0 / (1.0 * c->Nx * c->Ny * c->Nz)) : (TB)1;
         const int nb = (int)nblk(npairs, P);
-        hipLaunchKernelGGL((k_z_c2r<TB, NZ, P>), xgrid(nb), dim3(P * ZPlan<NZ>::TPP), 0, s, tmp, out, npairs,
+        hipLaunchKernelGGL((k_z_c2r<TB, NZ, P>), xgrid(nb), dim3(P * ZPlanT<TB, NZ>::TPP), 0, s, tmp, out, npairs,
                            c->Mz, nb, c->tab, twp<TB>(c, 2), sc);
         return NS3D_OK;
     });
@@ -636,7 +641,7 @@ int r2c_3d(ns3d_ctx* c, const TB* in, cplx<TB>* out) {
         constexpr int P = PZ<TB, NZ>;
         const long npairs = c->npen / 2;
         const int nb = (int)nblk(npairs, P);
-        hipLaunchKernelGGL((k_z_r2c<TB, NZ, P>), xgrid(nb), dim3(P * ZPlan<NZ>::TPP), 0, s, in, out, npairs,
+        hipLaunchKernelGGL((k_z_r2c<TB, NZ, P>), xgrid(nb), dim3(P * ZPlanT<TB, NZ>::TPP), 0, s, in, out, npairs,
                            c->Mz, nb, twp<TB>(c, 2));
         return NS3D_OK;
     });
