@@ -126,15 +126,16 @@ def cpu_baseline(N, u0, Re, dt, steps, threads):
                       f"(OpenMP, own mixed-radix FFT, not FFTW), {dt_s:.1f} s"}
 
 
-def run_serving(n_req, rate, seed, precision, max_active, chunk):
+def run_serving(n_req, rate, seed, precision, max_active, chunk, rm_fraction=0.15):
     """Served mixed-size stream (Poisson arrivals, N in {32..192}, Re ~ U(50,500),
-    T ~ U(0.5,5), a quarter J.v requests with T=1) through the continuous-batching
-    server, against one-at-a-time FP64 on the same requests."""
+    T ~ U(0.5,5); a quarter J.v requests with T=1 and a share of return-map
+    requests) through the continuous-batching server, against one-at-a-time FP64
+    on the same requests."""
     import dataclasses
     import torch
     from ns3dperiodic_amd import serve
     t0 = time.time()
-    reqs = serve.poisson_stream(n_req, rate, seed=seed)
+    reqs = serve.poisson_stream(n_req, rate, seed=seed, rm_fraction=rm_fraction)
     io = {r.rid: serve.pin_io(*serve.request_inputs(r)) for r in reqs}
     print(f"[serve] {n_req} requests generated in {time.time() - t0:.1f} s; total steps "
           f"{sum(r.nsteps for r in reqs)}", file=sys.stderr, flush=True)
@@ -168,7 +169,11 @@ def run_serving(n_req, rate, seed, precision, max_active, chunk):
     for d in by_n.values():
         d["fp64_ms_per_step"] = 1e3 * d["fp64_service_s"] / max(d["steps"], 1)
     errs, nonfinite = [], []
+    kinds = {k: sum(r.kind == k for r in reqs) for k in ("integrate", "jv", "return_map")}
+    rm_found = sum(1 for r in reqs if r.kind == "return_map" and res[r.rid].info and res[r.rid].info["found"])
     for r in reqs:
+        if r.kind == "return_map":
+            continue    # FP32 and FP64 may stop the search at different steps
         a, b = served[r.rid], base[r.rid].u
         if not all(np.isfinite(x).all() for x in a + b):
             nonfinite.append(r.rid)
@@ -190,7 +195,8 @@ def run_serving(n_req, rate, seed, precision, max_active, chunk):
             "speedup_solves_per_s": s_srv["solves_per_s"] / s_base["solves_per_s"],
             "max_rel_l2_vs_fp64": max(errs) if errs else None,
             "median_rel_l2_vs_fp64": float(np.median(errs)) if errs else None, "nonfinite": nonfinite,
-            "by_grid": {str(k): v for k, v in sorted(by_n.items())}}
+            "by_grid": {str(k): v for k, v in sorted(by_n.items())},
+            "kinds": kinds, "return_map_crossings_found": rm_found}
 
 
 def main():

@@ -25,47 +25,72 @@ def plane_value(p, x0, normal):
     return float(np.dot(normal, np.asarray(p) - np.asarray(x0)))
 
 
+class CrossingSearch:
+    """The crossing search one step at a time (for callers that interleave it
+    with other work).  ``advance()`` takes one step and returns True once the
+    crossing has been located; ``result`` then holds what find_crossing returns."""
+
+    def __init__(self, system, x0, normal, dt, rho=0.5, tol=1e-12, max_bisect=500, min_steps=3):
+        self.system = system
+        self.x0 = np.asarray(x0, float)
+        self.normal = np.asarray(normal, float)
+        self.dt, self.rho, self.tol, self.max_bisect, self.min_steps = dt, rho, tol, max_bisect, min_steps
+        self.prev = np.asarray(system.probe(), float)
+        self.t = 0.0
+        self.steps = 0
+        self.sizes = []
+        self.result = None
+
+    def advance(self):
+        sysm, x0, normal, dt = self.system, self.x0, self.normal, self.dt
+        back = sysm.save()
+        sysm.step(dt)
+        nxt = np.asarray(sysm.probe(), float)
+        s1, s2 = plane_value(self.prev, x0, normal), plane_value(nxt, x0, normal)
+        moving = float(np.dot(normal, nxt - self.prev))
+        if (self.steps > self.min_steps and np.linalg.norm(self.prev - x0) < self.rho and s1 * s2 < 0.0
+                and moving > 0.0):
+            # the crossing lies inside this step: bisect its length from the saved state
+            a, b = 0.0, dt
+            err, it, point, h = s2, 0, nxt, dt
+            while abs(err) > self.tol and it < self.max_bisect:
+                it += 1
+                h = 0.5 * (a + b)
+                sysm.restore(back)
+                sysm.step(h)
+                point = np.asarray(sysm.probe(), float)
+                err = plane_value(point, x0, normal)
+                if err > 0.0:
+                    b = h
+                else:
+                    a = h
+            # the system is left at the last sub-step evaluated
+            self.sizes.append(h)
+            self.result = {"found": True, "steps": self.steps + 1, "time": self.t + h, "point": point,
+                           "error": abs(err), "bisections": it, "step_sizes": self.sizes}
+            return True
+        self.sizes.append(dt)
+        self.t += dt
+        self.steps += 1
+        self.prev = nxt
+        return False
+
+    def not_found(self):
+        return {"found": False, "steps": self.steps, "time": self.t, "point": self.prev, "error": float("nan"),
+                "bisections": 0, "step_sizes": self.sizes}
+
+
 def find_crossing(system, x0, normal, dt, rho=0.5, max_steps=100000, tol=1e-12, max_bisect=500, min_steps=3):
     """Step `system` with dt until its probe crosses the plane (x0, normal) in
     the direction of `normal`; the system is left at the crossing.  Returns
     dict(found, steps, time, point, error, bisections, step_sizes): time is
     measured from the call, step_sizes lists every step length applied to the
     state that ends at the crossing (for replaying the solve)."""
-    x0 = np.asarray(x0, float)
-    normal = np.asarray(normal, float)
-    prev = np.asarray(system.probe(), float)
-    t = 0.0
-    sizes = []
-    for step in range(max_steps):
-        back = system.save()
-        system.step(dt)
-        nxt = np.asarray(system.probe(), float)
-        s1, s2 = plane_value(prev, x0, normal), plane_value(nxt, x0, normal)
-        moving = float(np.dot(normal, nxt - prev))
-        if step > min_steps and np.linalg.norm(prev - x0) < rho and s1 * s2 < 0.0 and moving > 0.0:
-            # the crossing lies inside this step: bisect its length from the saved state
-            a, b = 0.0, dt
-            err, it, point = s2, 0, nxt
-            while abs(err) > tol and it < max_bisect:
-                it += 1
-                mid = 0.5 * (a + b)
-                system.restore(back)
-                system.step(mid)
-                point = np.asarray(system.probe(), float)
-                err = plane_value(point, x0, normal)
-                if err > 0.0:
-                    b = mid
-                else:
-                    a = mid
-            # the system is left at the last sub-step evaluated
-            sizes.append(mid if it else dt)
-            return {"found": True, "steps": step + 1, "time": t + (mid if it else dt), "point": point,
-                    "error": abs(err), "bisections": it, "step_sizes": sizes}
-        sizes.append(dt)
-        t += dt
-        prev = nxt
-    return {"found": False, "steps": max_steps, "time": t, "point": prev, "error": float("nan"), "bisections": 0,
-            "step_sizes": sizes}
+    search = CrossingSearch(system, x0, normal, dt, rho, tol, max_bisect, min_steps)
+    for _ in range(max_steps):
+        if search.advance():
+            return search.result
+    return search.not_found()
 
 
 class NSSystem:

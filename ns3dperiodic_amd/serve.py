@@ -5,7 +5,12 @@ Kolmogorov-forced field to its horizon T with RK3_SSP
 (Source/RK_time_step.cu:310-327); ``jv`` carries a perturbation along the
 trajectory with the tangent map RK3_SSP_UV (Source/RK_time_step.cu:342-355),
 i.e. v(T) = dPhi_T(u0) v0, the product the reference's Krylov drivers build
-from finite differences of the time-T map.
+from finite differences of the time-T map; ``return_map`` steps until the
+velocity at a probe point returns through the plane through its initial
+value, normal to its initial rate of change (find_intersection,
+Source/return_map.cu:425-480), or until T.  Return-map requests probe after
+every step, so they advance one step per loop turn with a host
+synchronisation; the other kinds stream chunks.
 
 The server keeps every admitted request's state resident in HBM and admits
 new requests while others run, as soon as a slot and the HBM budget allow.
@@ -43,7 +48,7 @@ GRID_SIZES = (32, 48, 64, 96, 128, 192)
 @dataclasses.dataclass
 class Request:
     rid: int
-    kind: str          # "integrate" | "jv"
+    kind: str          # "integrate" | "jv" | "return_map"
     N: int             # cubic grid N^3 on [0, 2 pi]^3
     Re: float
     T: float           # horizon
@@ -61,6 +66,7 @@ class Result:
     latency: float
     u: list            # final state, host complex128 [N, N, N/2+1] x 3
     v: list = None     # final perturbation (jv)
+    info: dict = None  # return_map: found, time, point, steps, bisections
 
 
 def choose_dt(u_hat, T, N, cfl=0.2, L=2 * math.pi, fmax=0.0):
@@ -77,16 +83,17 @@ def choose_dt(u_hat, T, N, cfl=0.2, L=2 * math.pi, fmax=0.0):
 
 
 def poisson_stream(n, rate, seed=0, sizes=GRID_SIZES, re_range=(50.0, 500.0), t_range=(0.5, 5.0),
-                   jv_fraction=0.25, jv_T=1.0):
+                   jv_fraction=0.25, jv_T=1.0, rm_fraction=0.0):
     """Seeded request stream: exponential inter-arrival times at `rate` per
     second, N uniform over `sizes`, Re ~ U(re_range), T ~ U(t_range) for
-    integrate requests and T = jv_T for J.v requests."""
+    integrate and return-map requests and T = jv_T for J.v requests."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
     for i in range(n):
         t += float(rng.exponential(1.0 / rate)) if rate > 0 else 0.0
-        kind = "jv" if rng.random() < jv_fraction else "integrate"
+        x = rng.random()
+        kind = "jv" if x < jv_fraction else ("return_map" if x < jv_fraction + rm_fraction else "integrate")
         N = int(sizes[int(rng.integers(len(sizes)))])
         Re = float(rng.uniform(*re_range))
         T = float(rng.uniform(*t_range)) if kind == "integrate" else float(jv_T)
@@ -121,9 +128,12 @@ def field_bytes(N, precision):
 
 def request_bytes(req, precision):
     """field_bytes plus, for J.v, the perturbation (3 FP64 fields), the
-    tangent map's 6 FP64 stage fields and 15 more scratch fields (B(u,v))."""
+    tangent map's 6 FP64 stage fields and 15 more scratch fields (B(u,v));
+    for a return map, the saved state (3 FP64 fields)."""
     nc = req.N * req.N * (req.N // 2 + 1)
     b = field_bytes(req.N, precision)
+    if req.kind == "return_map":
+        b += nc * 16 * 3              # the state saved before each step (bisection restart)
     if req.kind == "jv":
         bulk = 16 if precision == "fp64" else 8
         b += nc * (16 * (3 + 6) + bulk * 15)
@@ -190,6 +200,21 @@ def summarize(results, t_first_arrival=0.0):
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99))}
 
 
+def probe_point(N):
+    """Grid point whose velocity a return-map request follows."""
+    return (N // 8, N // 4, N // 2)
+
+
+def _return_map_search(g, u, f, dt):
+    from .return_map import CrossingSearch, NSSystem
+    sysm = NSSystem(g, u, f, point=probe_point(g.Nx))
+    x0 = sysm.probe()
+    v = sysm.probe_rhs()
+    nrm = float(np.linalg.norm(v))
+    normal = v / nrm if nrm > 0 else np.array([1.0, 0.0, 0.0])
+    return CrossingSearch(sysm, x0, normal, dt)
+
+
 # ---------------------------------------------------------------------------- GPU side
 class _Slot:
     """A context, a stream and resident buffers for one request at a time;
@@ -225,6 +250,8 @@ class _Slot:
                     d.copy_(h, non_blocking=True)
         self.left = req.nsteps
         self.inflight = collections.deque()
+        self.search = None
+        self.info = None
 
     def issue(self):
         """Enqueue the next chunk of at most K steps; False once all are issued."""
@@ -232,7 +259,17 @@ class _Slot:
         if self.left <= 0:
             return False
         k = min(self.left, self.chunk)
-        if self.req.kind == "jv":
+        if self.req.kind == "return_map":
+            with torch.cuda.stream(self.stream):
+                if self.search is None:
+                    self.search = _return_map_search(self.g, self.u, self.f, self.req.dt)
+                k = 1
+                if self.search.advance():
+                    self.info = {k2: v for k2, v in self.search.result.items() if k2 != "step_sizes"}
+                    k = self.left
+                elif self.left == 1:
+                    self.info = {k2: v for k2, v in self.search.not_found().items() if k2 != "step_sizes"}
+        elif self.req.kind == "jv":
             for _ in range(k):
                 self.g.RK3_SSP_UV(self.u, self.v, self.f)
         elif self.use_graphs and k == self.chunk:
@@ -329,7 +366,7 @@ class Server:
                     r = s.req
                     out_v = [c.numpy() for c in s.io["out_v"]] if r.kind == "jv" else None
                     results[r.rid] = Result(r.rid, s.admitted, t, t - r.arrival,
-                                            [c.numpy() for c in s.io["out_u"]], out_v)
+                                            [c.numpy() for c in s.io["out_u"]], out_v, s.info)
                     s.done_ev = None
                     s.req = s.io = None
                     active.remove(s)
@@ -382,17 +419,28 @@ def one_at_a_time(requests, io, precision="fp64", device=0, clock=time.perf_coun
         inp = io[r.rid]
         u = [h.to(dev) for h in inp["u"]]
         f = [inp["f"].to(dev), None, None]
+        info = None
         if r.kind == "jv":
             v = [h.to(dev) for h in inp["v"]]
             for _ in range(r.nsteps):
                 g.RK3_SSP_UV(u, v, f)
+        elif r.kind == "return_map":
+            v = None
+            search = _return_map_search(g, u, f, r.dt)
+            found = False
+            for _ in range(r.nsteps):
+                if search.advance():
+                    found = True
+                    break
+            res = search.result if found else search.not_found()
+            info = {k: val for k, val in res.items() if k != "step_sizes"}
         else:
             v = None
             g.RK3_SSP(u, f, steps=r.nsteps)
         out_u = [c.cpu().numpy() for c in u]
         out_v = [c.cpu().numpy() for c in v] if v is not None else None
         t = clock() - t0
-        results[r.rid] = Result(r.rid, adm, t, t - r.arrival, out_u, out_v)
+        results[r.rid] = Result(r.rid, adm, t, t - r.arrival, out_u, out_v, info)
     if own:
         for g in ctxs.values():
             g.close()

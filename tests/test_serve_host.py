@@ -23,6 +23,18 @@ def test_poisson_stream_is_seeded_and_in_range():
     assert 0.1 < np.mean([r.kind == "jv" for r in a]) < 0.4
 
 
+def test_stream_with_return_map_requests():
+    a = serve.poisson_stream(400, rate=5.0, seed=4, jv_fraction=0.25, rm_fraction=0.25)
+    frac = {k: np.mean([r.kind == k for r in a]) for k in ("jv", "return_map", "integrate")}
+    assert abs(frac["jv"] - 0.25) < 0.07 and abs(frac["return_map"] - 0.25) < 0.07
+    assert all(0.5 <= r.T <= 5.0 for r in a if r.kind == "return_map")
+    r = next(q for q in a if q.kind == "return_map")
+    i = serve.Request(0, "integrate", r.N, 1.0, 1.0, 0, 0.0)
+    nc = r.N * r.N * (r.N // 2 + 1)
+    assert serve.request_bytes(r, "fp32") == serve.request_bytes(i, "fp32") + 48 * nc
+    assert all(0 <= p < r.N for p in serve.probe_point(r.N))
+
+
 def test_choose_dt_divides_horizon_and_respects_cfl():
     from ns3dperiodic_amd import fields
     N = 16
