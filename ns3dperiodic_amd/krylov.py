@@ -277,9 +277,13 @@ def _apply_shifts(H, shifts):
     return H, Q
 
 
-def eigs(ops, matvec, n, k=6, m=30, which="LM", tol=1e-10, max_iter=300, seed=0, v0=None, vectors=False):
+def eigs(ops, matvec, n, k=6, m=30, which="LM", tol=1e-10, max_iter=300, seed=0, v0=None, vectors=False, buffer=None):
     """k eigenvalues of the operator `matvec(x, y)` (y = A x, length-n FP64
-    vectors) with an implicitly restarted Arnoldi of m vectors.
+    vectors) with an implicitly restarted Arnoldi of m vectors.  Each restart
+    keeps k + buffer Ritz vectors (default buffer max(2, k/2)) and converges
+    the leading k: without the buffer an eigenvalue just below the k-th in the
+    `which` order can stay outside the subspace while a slightly smaller one
+    converges in its place.
 
     Returns dict(values, residuals, iterations, ritz_norm[, vectors_real,
     vectors_imag]): eigenvalues sorted by `which`, residuals ||A x - lambda x||
@@ -287,6 +291,8 @@ def eigs(ops, matvec, n, k=6, m=30, which="LM", tol=1e-10, max_iter=300, seed=0,
     """
     if not (0 < k < m <= 128 and m <= n):
         raise ValueError("need 0 < k < m <= min(n, 128)")
+    buffer = max(2, k // 2) if buffer is None else int(buffer)
+    kw = max(k, min(k + buffer, m - 3))
     V = ops.zeros(n, m)
     V1 = ops.zeros(n, m)
     H = np.zeros((m, m))
@@ -302,14 +308,16 @@ def eigs(ops, matvec, n, k=6, m=30, which="LM", tol=1e-10, max_iter=300, seed=0,
         arnoldi_extend(ops, matvec, V, H, f, p, m, w=w)
         vals = np.linalg.eigvals(H)
         order = _order(vals, which)
-        kk = k
+        kk = kw
         # keep a complex pair together
-        if abs(vals[order[k - 1]].imag) > 0 and kk < m - 1 and abs(vals[order[k]] - np.conj(vals[order[k - 1]])) <= 1e-10 * max(1.0, abs(vals[order[k - 1]])):
-            kk = k + 1
-        # Ritz residual estimates |beta_m e_m^T y|
+        if abs(vals[order[kk - 1]].imag) > 0 and kk < m - 1 and abs(vals[order[kk]] - np.conj(vals[order[kk - 1]])) <= 1e-10 * max(1.0, abs(vals[order[kk - 1]])):
+            kk = kk + 1
+        # Ritz residual estimates |beta_m e_m^T y| of the leading k (plus a split pair)
+        kc = k + 1 if (k < kk and abs(vals[order[k - 1]].imag) > 0
+                       and abs(vals[order[k]] - np.conj(vals[order[k - 1]])) <= 1e-10 * max(1.0, abs(vals[order[k - 1]]))) else k
         ev, Y = np.linalg.eig(H)
         fnorm = math.sqrt(max(ops.dot(f, f), 0.0))
-        want = order[:kk]
+        want = order[:kc]
         idx = [int(np.argmin(np.abs(ev - vals[i]))) for i in want]
         ritz = np.array([fnorm * abs(Y[m - 1, i]) / max(np.linalg.norm(Y[:, i]), 1e-300) for i in idx])
         scale = np.maximum(np.abs(vals[want]), 1e-300)
@@ -487,7 +495,12 @@ class NSOperators:
       (NSCallMatrixVector_reduced, :340-361);
     * ``rhs``: F(x) = L(x), the RK right-hand side of the state x
       (NSCallRHS_reduced, :414-440);
-    * ``jacobian(x)``: J(x), the base set to x (Newton).
+    * ``jacobian(x)``: J(x), the base set to x (Newton);
+    * ``tangent(steps)``: v -> dPhi(u0) v over `steps` tangent-map steps
+      RK3_SSP_UV (Source/RK_time_step.cu:342-355) from the base state u0,
+      restarted from u0 at every call: the time-T return-map linearisation
+      whose eigenvalues are Floquet multipliers (NSCallMatrixVector_exponential,
+      Source/NS3D_periodic.cu:444-470, which instead keeps advancing the base).
     """
 
     def __init__(self, g, u_base=None, f=None):
@@ -518,3 +531,13 @@ class NSOperators:
     def jacobian(self, x):
         self.set_base(self.g.unpack_reduced(x))
         return self.jv
+
+    def tangent(self, steps):
+        def mv(vin, vout):
+            u = [c.clone() for c in self.u]
+            v = self.g.unpack_reduced(vin)
+            for _ in range(steps):
+                self.g.RK3_SSP_UV(u, v, self.f)
+            vout.copy_(self.g.pack_reduced(v))
+            return vout
+        return mv

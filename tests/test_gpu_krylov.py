@@ -97,3 +97,33 @@ def test_newton_to_laminar_kolmogorov(jac_precision):
     assert out["converged"]
     err = np.linalg.norm(ops.to_host(x) - xs) / np.linalg.norm(xs)
     assert err < 1e-9
+
+
+def test_return_map_multipliers_at_steady_state():
+    """Arnoldi on the tangent map over n steps at the laminar Kolmogorov state:
+    there the tangent step (Heun on v with J fixed) is I + dtJ + (dtJ)^2/2, so
+    the multipliers are p(dt*lambda)^n for the eigenvalues lambda of the dense
+    Jacobian assembled from the oracle."""
+    import torch
+    from ns3dperiodic_amd import NS3D, fields, krylov
+    from oracle.oracle import Oracle
+    print("\n[case] return-map multipliers, 8^3", flush=True)
+    N, Re, dt, n = 8, 10.0, 0.01, 20
+    ust, f = fields.reference_kolmogorov(N, n=1, Re=Re)
+    g = NS3D(N, Re=Re, dt=dt)
+    o = Oracle(N, Re=Re, dt=dt)
+    nv = g.vector_length_reduced()
+    J = np.empty((nv, nv))
+    e = np.zeros(nv)
+    for i in range(nv):
+        e[i] = 1.0
+        J[:, i] = o.jv_reduced(ust, e)
+        e[i] = 0.0
+    z = dt * np.linalg.eigvals(J)
+    mu = (1.0 + z + 0.5 * z * z) ** n
+    ref = mu[krylov._order(mu, "LM")][:6]
+    ops = krylov.NSOperators(g, [torch.from_numpy(c).cuda() for c in ust], f=[torch.from_numpy(c).cuda() for c in f])
+    out = krylov.eigs(ops.ops, ops.tangent(n), nv, k=6, m=30, which="LM", tol=1e-11, max_iter=300)
+    print(f"[krylov] multipliers {np.round(out['values'], 8)}", flush=True)
+    for m_ in ref:
+        assert np.min(np.abs(out["values"] - m_)) < 1e-8, (m_, out["values"])

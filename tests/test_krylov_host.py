@@ -100,3 +100,35 @@ def test_shift_application_keeps_hessenberg_similarity():
     assert np.allclose(Q.T @ Q, np.eye(m), atol=1e-12)
     assert np.allclose(Q.T @ H @ Q, H2, atol=1e-10)
     assert np.allclose(np.sort_complex(np.linalg.eigvals(H2)), np.sort_complex(vals), atol=1e-9)
+
+
+def test_eigs_close_magnitudes_need_the_restart_buffer():
+    """Spectrum modelled on the tangent map of the laminar Kolmogorov state:
+    leading multipliers in complex pairs whose magnitudes differ by < 1e-3."""
+    rng = np.random.default_rng(6)
+    n = 240
+    mags = [1.005614, 1.000034, 0.981687, 0.981096, 0.980199, 0.9795]
+    angs = [0.3323, 0.2990, 0.2263, 0.3298, 0.0, 0.41]
+    blocks = []
+    for r, t in zip(mags, angs):
+        if t == 0.0:
+            blocks.append(np.array([[r]]))
+            blocks.append(np.array([[r * 0.9999]]))
+        else:
+            blocks.append(r * np.array([[np.cos(t), -np.sin(t)], [np.sin(t), np.cos(t)]]))
+    d = sum(b.shape[0] for b in blocks)
+    bulk = rng.uniform(0.2, 0.9, n - d)
+    D = np.zeros((n, n))
+    i = 0
+    for b in blocks:
+        s = b.shape[0]
+        D[i:i + s, i:i + s] = b
+        i += s
+    D[i:, i:] = np.diag(bulk)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = Q @ D @ Q.T
+    ref = np.linalg.eigvals(A)
+    ref = ref[krylov._order(ref, "LM")][:6]
+    out = krylov.eigs(OPS, _mv(A), n, k=6, m=30, which="LM", tol=1e-11)
+    for lam in ref:
+        assert np.min(np.abs(out["values"] - lam)) < 1e-9, (lam, out["values"])
