@@ -50,6 +50,29 @@ bool size_ok(int n) {
 template <typename TB> constexpr int PXY0 = sizeof(TB) == 8 ? 8 : NS3D_PXY32;
 template <typename TB, int N, int E>
 constexpr int PXY = PXY0<TB> * XYPlan<N, E>::TPP <= 1024 ? PXY0<TB> : 1024 / XYPlan<N, E>::TPP;
+// Pencils per block of each fused x/y pass, by precision (measured at 256^3:
+// 1024-thread blocks, which fill a CU's 16 waves with one block, lose to 512).
+#ifndef NS3D_PXINV32
+#define NS3D_PXINV32 8
+#endif
+#ifndef NS3D_PYINV32
+#define NS3D_PYINV32 8
+#endif
+#ifndef NS3D_PYFWD32
+#define NS3D_PYFWD32 16
+#endif
+#ifndef NS3D_PXFWD32
+#define NS3D_PXFWD32 8
+#endif
+#ifndef NS3D_PXINV64
+#define NS3D_PXINV64 8
+#endif
+#ifndef NS3D_PYINV64
+#define NS3D_PYINV64 8
+#endif
+template <typename TB, int N, int E, int P32, int P64>
+constexpr int PXY_T = (sizeof(TB) == 8 ? P64 : P32) * XYPlan<N, E>::TPP <= 1024 ? (sizeof(TB) == 8 ? P64 : P32)
+                                                                             : 1024 / XYPlan<N, E>::TPP;
 // Points per thread of each x/y pass, by precision (measured at 256^3 on MI355X):
 // FP64 favours 4 points in the inverse passes, 8 in the forward ones; FP32 the mix below.
 #ifndef NS3D_EXINV32
@@ -58,10 +81,19 @@ constexpr int PXY = PXY0<TB> * XYPlan<N, E>::TPP <= 1024 ? PXY0<TB> : 1024 / XYP
 #ifndef NS3D_EYINV32
 #define NS3D_EYINV32 4
 #endif
-template <typename TB> constexpr int E_XINV = sizeof(TB) == 8 ? 4 : NS3D_EXINV32;
-template <typename TB> constexpr int E_YINV = sizeof(TB) == 8 ? 4 : NS3D_EYINV32;
+#ifndef NS3D_EXINV64
+#define NS3D_EXINV64 4
+#endif
+#ifndef NS3D_EYINV64
+#define NS3D_EYINV64 4
+#endif
+#ifndef NS3D_EXFWD64
+#define NS3D_EXFWD64 8
+#endif
+template <typename TB> constexpr int E_XINV = sizeof(TB) == 8 ? NS3D_EXINV64 : NS3D_EXINV32;
+template <typename TB> constexpr int E_YINV = sizeof(TB) == 8 ? NS3D_EYINV64 : NS3D_EYINV32;
 template <typename TB> constexpr int E_C2C = 8;
-template <typename TB> constexpr int E_XFWD = sizeof(TB) == 8 ? 8 : 4;
+template <typename TB> constexpr int E_XFWD = sizeof(TB) == 8 ? NS3D_EXFWD64 : 4;
 // one output field per block in the inverse passes (input re-read by 2 or 3 blocks)
 #ifndef NS3D_OS_XINV
 #define NS3D_OS_XINV 0
@@ -358,7 +390,7 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
     st = with_size(c->Nx, [&](auto NXc) -> int {
         constexpr int NX = decltype(NXc)::value;
         constexpr int E = E_XINV<TB>;
-        constexpr int P = PXY<TB, NX, E>;
+        constexpr int P = PXY_T<TB, NX, E, NS3D_PXINV32, NS3D_PXINV64>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Ny * nlt;
         constexpr int OS = NS3D_OS_XINV;
@@ -383,7 +415,7 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
     st = with_size(c->Ny, [&](auto NYc) -> int {
         constexpr int NY = decltype(NYc)::value;
         constexpr int E = E_YINV<TB>;
-        constexpr int P = PXY<TB, NY, E>;
+        constexpr int P = PXY_T<TB, NY, E, NS3D_PYINV32, NS3D_PYINV64>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Nx * nlt;
         constexpr int OS = NS3D_OS_YINV;
@@ -422,7 +454,7 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
     st = with_size(c->Ny, [&](auto NYc) -> int {
         constexpr int NY = decltype(NYc)::value;
         constexpr int E = E_C2C<TB>;
-        constexpr int P = PXY<TB, NY, E>;
+        constexpr int P = PXY_T<TB, NY, E, NS3D_PYFWD32, 8>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Nx * nlt;
         dim3 grid = xgrid(nb, 3), blk(P * XYPlan<NY, E>::TPP);
@@ -441,7 +473,7 @@ int run_x_forward_epilogue(ns3d_ctx* c, const EpiArgs& ea) {
     int st = with_size(c->Nx, [&](auto NXc) -> int {
         constexpr int NX = decltype(NXc)::value;
         constexpr int E = E_XFWD<TB>;
-        constexpr int P = PXY<TB, NX, E>;
+        constexpr int P = PXY_T<TB, NX, E, NS3D_PXFWD32, 8>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Ny * nlt;
         dim3 grid = xgrid(nb), blk(P * XYPlan<NX, E>::TPP);
