@@ -86,7 +86,8 @@ def poisson_stream(n, rate, seed=0, sizes=GRID_SIZES, re_range=(50.0, 500.0), t_
                    jv_fraction=0.25, jv_T=1.0, rm_fraction=0.0):
     """Seeded request stream: exponential inter-arrival times at `rate` per
     second, N uniform over `sizes`, Re ~ U(re_range), T ~ U(t_range) for
-    integrate and return-map requests and T = jv_T for J.v requests."""
+    integrate requests and T = jv_T for J.v and return-map requests (a
+    return-map request stops at its crossing or at T)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
