@@ -114,6 +114,71 @@ def run_gpu(N, precision, steps, warmup, u0, Re, dt):
     }
 
 
+def run_scalar(N, steps, warmup, u0, theta0, Re, dt):
+    """RK3 of the velocity plus a WENO5-advected passive scalar (FP64), timed
+    like run_gpu; algorithmic bytes 159 S + 26 R per step (R = one physical
+    field: the stage's velocity written by the z pass (3), read by the scalar
+    update with theta_s (3 + 1), theta written (1), theta^n in stages 2-3)."""
+    import torch
+    from ns3dperiodic_amd import NS3D
+
+    g = NS3D(N, Re=Re, dt=dt)
+    u = [torch.from_numpy(c).to("cuda:0") for c in u0]
+    th = torch.from_numpy(theta0).to("cuda:0")
+    g.RK3_SSP_scalar(u, th, None, steps=warmup)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    g.RK3_SSP_scalar(u, th, None, steps=steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    total_ms = e0.elapsed_time(e1)
+    finite = bool(torch.isfinite(th).all().item())
+    g.close()
+    nc = N * N * (N // 2 + 1)
+    sb = step_bytes(nc, 16) + 26 * 8 * N ** 3
+    ms_step = total_ms / steps
+    return {"steps_per_s": steps / (total_ms * 1e-3), "ms_per_step": ms_step, "step_alg_bytes": sb,
+            "step_roofline_frac": sb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "finite": finite}
+
+
+def pad_spectrum(u, M):
+    """The same physical field on an M^3 grid (M > N): the N^3 half spectrum
+    zero-padded and scaled by (M/N)^3 (the inverse transform carries 1/M^3);
+    the N/2 Nyquist planes are dropped."""
+    N = u[0].shape[0]
+    h = N // 2
+    ti = np.r_[0:h, M - h + 1:M]      # target index of wavenumbers 0..h-1 and -(h-1)..-1
+    si = np.r_[0:h, h + 1:N]
+    out = []
+    for c in u:
+        o = np.zeros((M, M, M // 2 + 1), np.complex128)
+        o[np.ix_(ti, ti, np.arange(h))] = (M / N) ** 3 * c[np.ix_(si, si, np.arange(h))]
+        out.append(o)
+    return out
+
+
+def copy_peak_gbs(reps=5):
+    """Bandwidth of a plain 1 GiB device-to-device copy (read + write bytes / time)."""
+    import torch
+    n = (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return {"bytes_read": n * 8, "bytes_written": n * 8, "avg_ms": ms, "GBps": 2 * n * 8 / (ms * 1e-3) / 1e9}
+
+
 def cpu_baseline(N, u0, Re, dt, steps, threads):
     from oracle.oracle import Oracle
     o = Oracle(N, Re=Re, dt=dt, nthreads=threads)
@@ -199,6 +264,10 @@ def run_serving(n_req, rate, seed, precision, max_active, chunk, rm_fraction=0.1
             "kinds": kinds, "return_map_crossings_found": rm_found}
 
 
+def names_of(precision):
+    return {"fp64": "fp64", "fp32": "fp32_bulk", "fp32s": "fp32_state"}[precision]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +288,8 @@ def main():
     ap.add_argument("--serve-chunk", type=int, default=16, help="RK3 steps per captured graph")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (3 times): the reference kernel for HBM counters")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the 512^3 leg, the passive-scalar leg and the copy-peak measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
@@ -226,20 +297,9 @@ def main():
         sys.exit(2)
 
     if args.copy_calibration:
-        import torch
-        n = (1 << 30) // 8
-        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(3):
-            b.copy_(a)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / 3
-        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "avg_ms": ms,
-                          "GBps": 2 * n * 8 / (ms * 1e-3) / 1e9}))
+        c = copy_peak_gbs(reps=3)
+        print(json.dumps({"copy_bytes_read": c["bytes_read"], "copy_bytes_written": c["bytes_written"],
+                          "avg_ms": c["avg_ms"], "GBps": c["GBps"]}))
         return
 
     from ns3dperiodic_amd import fields
@@ -260,6 +320,30 @@ def main():
             continue
         secs.append(run_gpu(N, p, args.steps, args.warmup, u0, Re, dt))
         print(f"[bench] {p}: {secs[-1]['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+
+    extra = None
+    if not args.no_extra:
+        extra = {"copy_peak": copy_peak_gbs()}
+        print(f"[bench] copy peak {extra['copy_peak']['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
+        th0 = fields.random_scalar(N, seed=args.seed)
+        sc = run_scalar(N, max(args.steps // 4, 5), 2, u0, th0, Re, dt)
+        extra["scalar"] = dict(sc, workload=f"{N}^3 FP64 velocity + WENO5 passive scalar "
+                                            "(E_theta ~ k^2 exp(-(k/4)^2), unit variance)")
+        print(f"[bench] fp64 + scalar: {sc['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+        if N == 256:
+            M = 512
+            u512 = pad_spectrum(u0, M)
+            dt512 = float("%.1e" % (dt * N / M))
+            big = {"workload": f"the {N}^3 field zero-padded to {M}^3, same spectrum, dt = {dt512:g}", "dt": dt512}
+            for p in ["fp64", "fp32", "fp32s"]:
+                r = run_gpu(M, p, 5, 1, u512, Re, dt512)
+                big[names_of(p)] = {"steps_per_s": r["steps_per_s"], "ms_per_step": r["ms_per_step"],
+                                    "step_alg_bytes": r["step_alg_bytes"],
+                                    "step_roofline_frac": r["step_GBps"] / HBM_PEAK_GBS, "finite": r["finite"],
+                                    "passes_avg_ms": {q["pass"]: q["avg_ms"] for q in r["passes"]}}
+                print(f"[bench] {M}^3 {p}: {r['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+            del u512
+            extra["n512"] = big
 
     dom = max(prim["passes"], key=lambda p: p["total_ms"])
     traffic = profiled_traffic(args.precision, dom["kernel"]) if N == 256 else None
@@ -303,9 +387,15 @@ def main():
         "finite": prim["finite"],
         "serving": serving,
     }
-    names = {"fp64": "fp64", "fp32": "fp32_bulk", "fp32s": "fp32_state"}
+    if extra is not None:
+        cp = extra["copy_peak"]["GBps"]
+        out["copy_peak"] = dict(extra["copy_peak"], step_frac_of_copy=prim["step_GBps"] / cp,
+                                dominant_frac_of_copy=dom["GBps"] / cp)
+        out["scalar"] = extra["scalar"]
+        if "n512" in extra:
+            out["n512"] = extra["n512"]
     for sec in secs:
-        out[names[sec["precision"]]] = {
+        out[names_of(sec["precision"])] = {
             "steps_per_s": sec["steps_per_s"], "ms_per_step": sec["ms_per_step"],
             "speedup_vs_primary": sec["steps_per_s"] / prim["steps_per_s"],
             "step_roofline_frac": sec["step_GBps"] / HBM_PEAK_GBS,

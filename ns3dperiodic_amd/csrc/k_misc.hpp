@@ -175,6 +175,18 @@ k_pack_reduced_shifted(PtrArr<const cplx<double>, 3> v, const double* __restrict
 
 // ------------------------------------------------------- a9: WENO advection
 // Reference: Source/advection_WENO.cu:47-101 (weno5/weno3), :105-301 (faces), :304-360 (flux).
+// The nonlinear weights alpha_k = c_k / (eps + beta_k)^2 normalised by their
+// sum are formed over the common denominator prod_k (eps + beta_k)^2, so a
+// face costs one FP64 division instead of six; if those products leave the
+// FP64 range (smoothness indicators beyond ~1e70) the per-weight form is used.
+__device__ __forceinline__ double weno_combine3(double c0, double c1, double c2, double d0, double d1, double d2,
+                                                double q0, double q1, double q2) {
+    const double a0 = c0 * (d1 * d2), a1 = c1 * (d0 * d2), a2 = c2 * (d0 * d1);
+    const double s = a0 + a1 + a2;
+    if (s < 1.0e300 && s > 1.0e-300) return (a0 * q0 + a1 * q1 + a2 * q2) / s;
+    const double b0 = c0 / d0, b1 = c1 / d1, b2 = c2 / d2, t = b0 + b1 + b2;
+    return b0 / t * q0 + b1 / t * q1 + b2 / t * q2;
+}
 __device__ __forceinline__ double weno5(double f1, double f2, double f3, double f4, double f5) {
     const double eps = 1.0E-22;
     const double q0 = 1.0 / 3.0 * f1 - 7.0 / 6.0 * f2 + 11.0 / 6.0 * f3;
@@ -186,20 +198,18 @@ __device__ __forceinline__ double weno5(double f1, double f2, double f3, double 
     const double b0 = 13.0 / 12.0 * t0 * t0 + 0.25 * u0 * u0;
     const double b1 = 13.0 / 12.0 * t1 * t1 + 0.25 * u1 * u1;
     const double b2 = 13.0 / 12.0 * t2 * t2 + 0.25 * u2 * u2;
-    const double a0 = 0.1 / ((eps + b0) * (eps + b0));
-    const double a1 = 0.6 / ((eps + b1) * (eps + b1));
-    const double a2 = 0.3 / ((eps + b2) * (eps + b2));
-    const double s = a0 + a1 + a2;
-    return a0 / s * q0 + a1 / s * q1 + a2 / s * q2;
+    return weno_combine3(0.1, 0.6, 0.3, (eps + b0) * (eps + b0), (eps + b1) * (eps + b1), (eps + b2) * (eps + b2),
+                         q0, q1, q2);
 }
 __device__ __forceinline__ double weno3(double f1, double f2, double f3) {
     const double eps = 1.0E-22;
     const double q0 = -0.5 * f1 + 1.5 * f2, q1 = 0.5 * f2 + 0.5 * f3;
     const double b0 = (f2 - f1) * (f2 - f1), b1 = (f3 - f2) * (f3 - f2);
-    const double a0 = (1.0 / 3.0) / ((eps + b0) * (eps + b0));
-    const double a1 = (2.0 / 3.0) / ((eps + b1) * (eps + b1));
-    const double s = a0 + a1;
-    return a0 / s * q0 + a1 / s * q1;
+    const double d0 = (eps + b0) * (eps + b0), d1 = (eps + b1) * (eps + b1);
+    const double a0 = (1.0 / 3.0) * d1, a1 = (2.0 / 3.0) * d0, s = a0 + a1;
+    if (s < 1.0e300 && s > 1.0e-300) return (a0 * q0 + a1 * q1) / s;
+    const double c0 = (1.0 / 3.0) / d0, c1 = (2.0 / 3.0) / d1, t = c0 + c1;
+    return c0 / t * q0 + c1 / t * q1;
 }
 
 struct Grid3 {
@@ -212,80 +222,102 @@ struct Grid3 {
     }
 };
 
-// Face value of x at cell (j,k,l) along axis AX, left- or right-biased.
-template <int AX>
-__device__ __forceinline__ double weno_face(const double* __restrict__ x, const Grid3& G, int j, int k, int l,
-                                            bool right, int scheme, int zfix) {
-    auto X = [&](int o) {
-        return AX == 0 ? x[G.at(j + o, k, l)] : AX == 1 ? x[G.at(j, k + o, l)] : x[G.at(j, k, l + o)];
-    };
-    if (scheme == 3) {
-        const double f1 = X(1), f2 = X(0), f3 = X(-1);
-        return right ? weno3(f1, f2, f3) : weno3(f3, f2, f1);
+// The two faces of a cell along one axis, upwinded by the face velocities
+// Um (face -1/2) and Up (face +1/2): flux_{x,y,z}_weno_{L,R}
+// (Source/advection_WENO.cu:105-301).  w[o] is the scalar at offset o along
+// the axis (|o| <= 3 for WENO5, 2 for WENO3, 1 for the first-order scheme).
+// Each face takes one reconstruction, its stencil selected by the upwind
+// side, so diverging lanes run no second branch.  q9: the z right-biased
+// reconstruction reads offset -1 where -2 belongs (Source/advection_WENO.cu:288).
+template <int SCHEME>
+__device__ __forceinline__ void weno_faces(const double* w, double Um, double Up, bool q9, double& sm, double& sp) {
+    const bool lm = Um >= 0.0, lp = Up >= 0.0;
+    if constexpr (SCHEME == 5) {
+        sm = weno5(lm ? w[-3] : w[2], lm ? w[-2] : w[1], lm ? w[-1] : w[0], lm ? w[0] : w[-1],
+                   lm ? w[1] : (q9 ? w[-1] : w[-2]));
+        sp = weno5(lp ? w[-2] : w[3], lp ? w[-1] : w[2], lp ? w[0] : w[1], lp ? w[1] : w[0],
+                   lp ? w[2] : (q9 ? w[0] : w[-1]));
+    } else if constexpr (SCHEME == 3) {
+        sm = weno3(lm ? w[-2] : w[1], lm ? w[-1] : w[0], lm ? w[0] : w[-1]);
+        sp = weno3(lp ? w[-1] : w[2], lp ? w[0] : w[1], lp ? w[1] : w[0]);
+    } else {
+        sm = lm ? w[-1] : w[0];
+        sp = lp ? w[0] : w[1];
     }
-    if (scheme == 5) {
-        const double f1 = X(2), f2 = X(1), f3 = X(0), f4 = X(-1);
-        // Q9: flux_z_weno_R reads l-1 as its fifth point (Source/advection_WENO.cu:288)
-        const double f5 = (AX == 2 && right && !zfix) ? X(-1) : X(-2);
-        return right ? weno5(f1, f2, f3, f4, f5) : weno5(f5, f4, f3, f2, f1);
-    }
-    return X(0);
 }
 
+__device__ __forceinline__ int wrap_add(int i, int o, int n) {
+    const int v = i + o;
+    return v < 0 ? v + n : (v >= n ? v - n : v);
+}
+
+// (u.grad) x at (j,k,l) with upwind WENO faces chosen by the sign of the
+// face-averaged velocity: flux_device + advection (Source/advection_WENO.cu:304-392).
+template <int SCHEME, typename TU>
+__device__ __forceinline__ double weno_point(const TU* __restrict__ ux, const TU* __restrict__ uy,
+                                             const TU* __restrict__ uz, const double* __restrict__ x, const Grid3& G,
+                                             long id, int j, int k, int l, double rdx, double rdy, double rdz, int zfix) {
+    constexpr int R = SCHEME == 5 ? 3 : (SCHEME == 3 ? 2 : 1);
+    const long sx = (long)G.Ny * G.Nz, sy = G.Nz;
+    const long bx = id - (long)j * sx, by = id - (long)k * sy, bz = id - l;   // id with j, k or l removed
+    long ox[2 * R + 1], oy[2 * R + 1], oz[2 * R + 1];
+#pragma unroll
+    for (int o = -R; o <= R; o++) {
+        ox[o + R] = bx + wrap_add(j, o, G.Nx) * sx;
+        oy[o + R] = by + (long)wrap_add(k, o, G.Ny) * sy;
+        oz[o + R] = bz + wrap_add(l, o, G.Nz);
+    }
+    const double uc = (double)ux[id], vc = (double)uy[id], wc = (double)uz[id];
+    const double Uw = 0.5 * ((double)ux[ox[R - 1]] + uc), Ue = 0.5 * (uc + (double)ux[ox[R + 1]]);
+    const double Un = 0.5 * ((double)uy[oy[R - 1]] + vc), Us = 0.5 * (vc + (double)uy[oy[R + 1]]);
+    const double Ur = 0.5 * ((double)uz[oz[R - 1]] + wc), Uf = 0.5 * (wc + (double)uz[oz[R + 1]]);
+    double wx[2 * R + 1], wy[2 * R + 1], wz[2 * R + 1];
+    const double xc = x[id];
+#pragma unroll
+    for (int o = 0; o <= 2 * R; o++) {
+        wx[o] = o == R ? xc : x[ox[o]];
+        wy[o] = o == R ? xc : x[oy[o]];
+        wz[o] = o == R ? xc : x[oz[o]];
+    }
+    double sw, se, sn, ss, sr, sf;
+    weno_faces<SCHEME>(wx + R, Uw, Ue, false, sw, se);
+    weno_faces<SCHEME>(wy + R, Un, Us, false, sn, ss);
+    weno_faces<SCHEME>(wz + R, Ur, Uf, !zfix, sr, sf);
+    return uc * (se - sw) * rdx + vc * (ss - sn) * rdy + wc * (sf - sr) * rdz;
+}
+
+// Point index of this thread: blocks are remapped so that each XCD (block b
+// runs on XCD b mod 8) sweeps one contiguous run of x-planes, keeping the
+// +-3-plane stencil reuse inside one L2.  The grid is a multiple of 8 blocks.
+__device__ __forceinline__ long weno_index() { return (long)xcd_tile() * blockDim.x + threadIdx.x; }
+
 // advection kernel + flux_device (Source/advection_WENO.cu:304-392), generalised to any advected scalar x.
+template <int SCHEME>
 __global__ void __launch_bounds__(256)
 k_weno(const double* __restrict__ ux, const double* __restrict__ uy, const double* __restrict__ uz,
-       const double* __restrict__ x, double* __restrict__ out, Grid3 G, double dx, double dy, double dz,
-       int scheme, int zfix) {
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+       const double* __restrict__ x, double* __restrict__ out, Grid3 G, double rdx, double rdy, double rdz, int zfix) {
+    const long id = weno_index();
     const long n = (long)G.Nx * G.Ny * G.Nz;
     if (id >= n) return;
     const int l = (int)(id % G.Nz);
     const long t1 = id / G.Nz;
     const int k = (int)(t1 % G.Ny), j = (int)(t1 / G.Ny);
-    const double Uw = 0.5 * (ux[G.at(j - 1, k, l)] + ux[id]);
-    const double Ue = 0.5 * (ux[id] + ux[G.at(j + 1, k, l)]);
-    const double sw = Uw >= 0.0 ? weno_face<0>(x, G, j - 1, k, l, false, scheme, zfix) : weno_face<0>(x, G, j, k, l, true, scheme, zfix);
-    const double se = Ue >= 0.0 ? weno_face<0>(x, G, j, k, l, false, scheme, zfix) : weno_face<0>(x, G, j + 1, k, l, true, scheme, zfix);
-    const double Un = 0.5 * (uy[G.at(j, k - 1, l)] + uy[id]);
-    const double Us = 0.5 * (uy[id] + uy[G.at(j, k + 1, l)]);
-    const double sn = Un >= 0.0 ? weno_face<1>(x, G, j, k - 1, l, false, scheme, zfix) : weno_face<1>(x, G, j, k, l, true, scheme, zfix);
-    const double ss = Us >= 0.0 ? weno_face<1>(x, G, j, k, l, false, scheme, zfix) : weno_face<1>(x, G, j, k + 1, l, true, scheme, zfix);
-    const double Ur = 0.5 * (uz[G.at(j, k, l - 1)] + uz[id]);
-    const double Uf = 0.5 * (uz[id] + uz[G.at(j, k, l + 1)]);
-    const double sr = Ur >= 0.0 ? weno_face<2>(x, G, j, k, l - 1, false, scheme, zfix) : weno_face<2>(x, G, j, k, l, true, scheme, zfix);
-    const double sf = Uf >= 0.0 ? weno_face<2>(x, G, j, k, l, false, scheme, zfix) : weno_face<2>(x, G, j, k, l + 1, true, scheme, zfix);
-    out[id] = ux[id] * (se - sw) / dx + uy[id] * (ss - sn) / dy + uz[id] * (sf - sr) / dz;
+    out[id] = weno_point<SCHEME>(ux, uy, uz, x, G, id, j, k, l, rdx, rdy, rdz, zfix);
 }
 
 // SSP stage of a passive scalar: out = a0*th_n + a1*(th_s - dt * (u.grad)_WENO th_s),
 // u being the masked physical velocity of the same stage (TU = bulk precision).
-template <typename TU>
+template <int SCHEME, typename TU>
 __global__ void __launch_bounds__(256)
 k_weno_stage(const TU* __restrict__ u, long nr, const double* __restrict__ x, const double* __restrict__ xn,
-             double* __restrict__ out, Grid3 G, double dx, double dy, double dz, int scheme, int zfix, double dt,
+             double* __restrict__ out, Grid3 G, double rdx, double rdy, double rdz, int zfix, double dt,
              double a0, double a1) {
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long id = weno_index();
     if (id >= nr) return;
-    const TU* ux = u;
-    const TU* uy = u + nr;
-    const TU* uz = u + 2 * nr;
     const int l = (int)(id % G.Nz);
     const long t1 = id / G.Nz;
     const int k = (int)(t1 % G.Ny), j = (int)(t1 / G.Ny);
-    const double Uw = 0.5 * ((double)ux[G.at(j - 1, k, l)] + (double)ux[id]);
-    const double Ue = 0.5 * ((double)ux[id] + (double)ux[G.at(j + 1, k, l)]);
-    const double sw = Uw >= 0.0 ? weno_face<0>(x, G, j - 1, k, l, false, scheme, zfix) : weno_face<0>(x, G, j, k, l, true, scheme, zfix);
-    const double se = Ue >= 0.0 ? weno_face<0>(x, G, j, k, l, false, scheme, zfix) : weno_face<0>(x, G, j + 1, k, l, true, scheme, zfix);
-    const double Un = 0.5 * ((double)uy[G.at(j, k - 1, l)] + (double)uy[id]);
-    const double Us = 0.5 * ((double)uy[id] + (double)uy[G.at(j, k + 1, l)]);
-    const double sn = Un >= 0.0 ? weno_face<1>(x, G, j, k - 1, l, false, scheme, zfix) : weno_face<1>(x, G, j, k, l, true, scheme, zfix);
-    const double ss = Us >= 0.0 ? weno_face<1>(x, G, j, k, l, false, scheme, zfix) : weno_face<1>(x, G, j, k + 1, l, true, scheme, zfix);
-    const double Ur = 0.5 * ((double)uz[G.at(j, k, l - 1)] + (double)uz[id]);
-    const double Uf = 0.5 * ((double)uz[id] + (double)uz[G.at(j, k, l + 1)]);
-    const double sr = Ur >= 0.0 ? weno_face<2>(x, G, j, k, l - 1, false, scheme, zfix) : weno_face<2>(x, G, j, k, l, true, scheme, zfix);
-    const double sf = Uf >= 0.0 ? weno_face<2>(x, G, j, k, l, false, scheme, zfix) : weno_face<2>(x, G, j, k, l + 1, true, scheme, zfix);
-    const double adv = (double)ux[id] * (se - sw) / dx + (double)uy[id] * (ss - sn) / dy + (double)uz[id] * (sf - sr) / dz;
+    const double adv = weno_point<SCHEME>(u, u + nr, u + 2 * nr, x, G, id, j, k, l, rdx, rdy, rdz, zfix);
     const double e = x[id] - dt * adv;
     out[id] = xn ? a0 * xn[id] + a1 * e : a1 * e;
 }

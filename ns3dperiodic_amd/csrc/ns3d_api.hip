@@ -767,16 +767,24 @@ RedArgs red_args(const ns3d_ctx* c, const double* a, const double* b, const doub
 }
 
 // ---------------------------------------------------------------- passive scalar RK3
-int weno_stage(ns3d_ctx* c, int scheme, const double* xs, const double* xn, double* out, double a0, double a1) {
-    Grid3 G{c->Nx, c->Ny, c->Nz};
+template <int SCHEME, typename TU>
+void launch_weno_stage(ns3d_ctx* c, const double* xs, const double* xn, double* out, double a0, double a1) {
+    const Grid3 G{c->Nx, c->Ny, c->Nz};
     const int zfix = (c->flags & NS3D_FLAG_WENO_ZFIX) ? 1 : 0;
-    const double dx = c->Lx / c->Nx, dy = c->Ly / c->Ny, dz = c->Lz / c->Nz;
-    if (c->precision == NS3D_FP64)
-        hipLaunchKernelGGL(k_weno_stage<double>, dim3(nblk(c->nr, 256)), dim3(256), 0, c->stream, (const double*)c->uphys,
-                           c->nr, xs, xn, out, G, dx, dy, dz, scheme, zfix, c->dt, a0, a1);
-    else
-        hipLaunchKernelGGL(k_weno_stage<float>, dim3(nblk(c->nr, 256)), dim3(256), 0, c->stream, (const float*)c->uphys,
-                           c->nr, xs, xn, out, G, dx, dy, dz, scheme, zfix, c->dt, a0, a1);
+    hipLaunchKernelGGL((k_weno_stage<SCHEME, TU>), xgrid(nblk(c->nr, 256)), dim3(256), 0, c->stream,
+                       (const TU*)c->uphys, c->nr, xs, xn, out, G, c->Nx / c->Lx, c->Ny / c->Ly, c->Nz / c->Lz, zfix,
+                       c->dt, a0, a1);
+}
+template <typename TU>
+void launch_weno_stage_s(ns3d_ctx* c, int scheme, const double* xs, const double* xn, double* out, double a0,
+                         double a1) {
+    if (scheme == 5) launch_weno_stage<5, TU>(c, xs, xn, out, a0, a1);
+    else if (scheme == 3) launch_weno_stage<3, TU>(c, xs, xn, out, a0, a1);
+    else launch_weno_stage<0, TU>(c, xs, xn, out, a0, a1);
+}
+int weno_stage(ns3d_ctx* c, int scheme, const double* xs, const double* xn, double* out, double a0, double a1) {
+    if (c->precision == NS3D_FP64) launch_weno_stage_s<double>(c, scheme, xs, xn, out, a0, a1);
+    else launch_weno_stage_s<float>(c, scheme, xs, xn, out, a0, a1);
     return hipGetLastError() == hipSuccess ? NS3D_OK : NS3D_ERR_HIP;
 }
 
@@ -1195,9 +1203,16 @@ int ns3d_divergence_physical(ns3d_ctx* c, const ns3d_cplx* ux, const ns3d_cplx* 
 int ns3d_weno_advect(ns3d_ctx* c, int scheme, const double* ux, const double* uy, const double* uz, const double* x,
                      double* out) {
     if (!c || !ux || !uy || !uz || !x || !out) return NS3D_ERR_ARG;
-    Grid3 G{c->Nx, c->Ny, c->Nz};
-    hipLaunchKernelGGL(k_weno, dim3(nblk(c->nr, 256)), dim3(256), 0, c->stream, ux, uy, uz, x, out, G, c->Lx / c->Nx,
-                       c->Ly / c->Ny, c->Lz / c->Nz, scheme, (c->flags & NS3D_FLAG_WENO_ZFIX) ? 1 : 0);
+    const Grid3 G{c->Nx, c->Ny, c->Nz};
+    const int zfix = (c->flags & NS3D_FLAG_WENO_ZFIX) ? 1 : 0;
+    const double rdx = c->Nx / c->Lx, rdy = c->Ny / c->Ly, rdz = c->Nz / c->Lz;
+    const dim3 grid = xgrid(nblk(c->nr, 256));
+    if (scheme == 5)
+        hipLaunchKernelGGL(k_weno<5>, grid, dim3(256), 0, c->stream, ux, uy, uz, x, out, G, rdx, rdy, rdz, zfix);
+    else if (scheme == 3)
+        hipLaunchKernelGGL(k_weno<3>, grid, dim3(256), 0, c->stream, ux, uy, uz, x, out, G, rdx, rdy, rdz, zfix);
+    else
+        hipLaunchKernelGGL(k_weno<0>, grid, dim3(256), 0, c->stream, ux, uy, uz, x, out, G, rdx, rdy, rdz, zfix);
     HIPCHK(hipGetLastError());
     return NS3D_OK;
 }
