@@ -48,10 +48,15 @@ bool size_ok(int n) {
 #define NS3D_PXY32 16
 #endif
 template <typename TB> constexpr int PXY0 = sizeof(TB) == 8 ? 8 : NS3D_PXY32;
+// Blocks are capped at NS3D_XY_MAXT threads: 1024-thread blocks, which fill
+// a CU's 16 waves with one block, lose to 512 (measured at 256^3 and 512^3).
+#ifndef NS3D_XY_MAXT
+#define NS3D_XY_MAXT 512
+#endif
 template <typename TB, int N, int E>
-constexpr int PXY = PXY0<TB> * XYPlan<N, E>::TPP <= 1024 ? PXY0<TB> : 1024 / XYPlan<N, E>::TPP;
-// Pencils per block of each fused x/y pass, by precision (measured at 256^3:
-// 1024-thread blocks, which fill a CU's 16 waves with one block, lose to 512).
+constexpr int PXY = PXY0<TB> * XYPlan<N, E>::TPP <= NS3D_XY_MAXT ? PXY0<TB>
+                                                                 : (NS3D_XY_MAXT / XYPlan<N, E>::TPP > 0 ? NS3D_XY_MAXT / XYPlan<N, E>::TPP : 1);
+// Pencils per block of each fused x/y pass, by precision.
 #ifndef NS3D_PXINV32
 #define NS3D_PXINV32 8
 #endif
@@ -71,8 +76,9 @@ constexpr int PXY = PXY0<TB> * XYPlan<N, E>::TPP <= 1024 ? PXY0<TB> : 1024 / XYP
 #define NS3D_PYINV64 8
 #endif
 template <typename TB, int N, int E, int P32, int P64>
-constexpr int PXY_T = (sizeof(TB) == 8 ? P64 : P32) * XYPlan<N, E>::TPP <= 1024 ? (sizeof(TB) == 8 ? P64 : P32)
-                                                                             : 1024 / XYPlan<N, E>::TPP;
+constexpr int PXY_T = (sizeof(TB) == 8 ? P64 : P32) * XYPlan<N, E>::TPP <= NS3D_XY_MAXT
+                          ? (sizeof(TB) == 8 ? P64 : P32)
+                          : (NS3D_XY_MAXT / XYPlan<N, E>::TPP > 0 ? NS3D_XY_MAXT / XYPlan<N, E>::TPP : 1);
 // Points per thread of each x/y pass, by precision (measured at 256^3 on MI355X):
 // FP64 favours 4 points in the inverse passes, 8 in the forward ones; FP32 the mix below.
 #ifndef NS3D_EXINV32
