@@ -237,6 +237,33 @@ def test_shapiro_config1():
     assert err < 1e-12
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-5), ("fp32s", 1e-5)])
+def test_shapiro_512(precision, tol):
+    """The largest grid: the Shapiro flow at 512^3 (an exact solution whose
+    nonlinear term is a pure gradient) stays on its analytic decay over 10
+    RK3 steps, through every 512-point plan of the fused passes.  Re = 500:
+    the explicit diffusion (Source/diffusion.cu:96-142) is stable only for
+    dt*|k|max^2/Re below ~2.5 (RK3's real-axis limit), and |k|max^2 = 3*256^2
+    here, so Re = 20 with dt = 1e-3 would amplify round-off in the highest
+    modes by ~100 per step."""
+    from tests.gpu_util import to_dev
+    from ns3dperiodic_amd import NS3D
+    import torch
+    print(f"\n[case] shapiro 512^3 {precision}", flush=True)
+    N, Re, dt, steps = 512, 500.0, 1e-3, 10
+    g = NS3D(N, Re=Re, dt=dt, precision=precision)
+    u = [g.FFTN(c) for c in to_dev(fields.shapiro(N))]
+    g.RK3_SSP(u, None, steps=steps)
+    ur = [g.iFFTN(c) for c in u]
+    del u
+    err = g.Shapiro_test_case(steps * dt, ur)
+    print(f"[shapiro] 512^3 {precision}: L-inf error after {steps} steps: {err:.3e}", flush=True)
+    g.close()
+    del ur
+    torch.cuda.empty_cache()
+    assert err < tol
+
+
 @pytest.mark.parametrize("shape", [(512, 16, 16), (16, 512, 16), (16, 16, 512), (16, 192, 96)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_large_axes(shape):
