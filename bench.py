@@ -143,6 +143,53 @@ def run_scalar(N, steps, warmup, u0, theta0, Re, dt):
             "step_roofline_frac": sb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "finite": finite}
 
 
+def run_fd_jv_batch(N=64, m=64, Re=150.0, T=1.0, eps=1e-7, seed=1):
+    """Batched Krylov work: finite-difference J.v of the time-T map for m
+    seeded unit-norm solenoidal perturbations of an N^3 Kolmogorov-forced
+    field (f_x = sin 4y, Re = 150, config-2 spectrum), all m + 1 fields
+    advanced together (ns3d_rk3_ssp_batch), against the same m + 1 solves
+    one field at a time.  FP64 (eps = 1e-7 is below FP32 resolution)."""
+    import torch
+    from ns3dperiodic_amd import NS3D, fields, krylov
+    u0 = fields.random_solenoidal(N, seed=seed, kind="k4exp", k0=4.0, energy=0.5)
+    f = fields.kolmogorov_forcing_sin(N, n=4, amp=1.0)
+    umax = max(float(np.abs(np.fft.irfftn(c, s=(N, N, N), axes=(0, 1, 2))).max()) for c in u0)
+    steps = int(math.ceil(T / (0.2 * (2 * math.pi / N) / (umax + T))))
+    dt = T / steps
+    vs = []
+    for i in range(m):
+        v = np.stack(fields.random_solenoidal(N, seed=1000 + i, kind="k4exp", k0=4.0, energy=0.5))
+        vs.append(v / np.sqrt(np.sum(np.abs(v) ** 2)))
+    V = torch.from_numpy(np.stack(vs)).cuda().contiguous()
+    u = [torch.from_numpy(c).cuda() for c in u0]
+    fd = [torch.from_numpy(c).cuda() for c in f]
+    g = NS3D(N, Re=Re, dt=dt)
+    krylov.fd_jv_batch(g, u, V[:2], eps=eps, steps=1, f=fd)     # warm-up (buffers, code objects)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = krylov.fd_jv_batch(g, u, V, eps=eps, steps=steps, f=fd)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    # one field at a time: the base and each perturbed field stepped alone
+    t0 = time.perf_counter()
+    base = [c.clone() for c in u]
+    g.RK3_SSP(base, fd, steps=steps)
+    same = True
+    for i in range(m):
+        w = [(u[k] + eps * V[i, k]).contiguous() for k in range(3)]
+        g.RK3_SSP(w, fd, steps=steps)
+        if i < 4:
+            same = same and all(torch.equal(J[i, k], (w[k] - base[k]) / eps) for k in range(3))
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    g.close()
+    return {"workload": f"{m} finite-difference J.v (eps={eps:g}) of the T={T:g} map at {N}^3, Re={Re:g}, "
+                        f"f_x = sin 4y, {steps} RK3 steps of dt={dt:.4g}; {m + 1} fields per batch",
+            "jv_per_s_batched": m / t_batch, "jv_per_s_one_at_a_time": m / t_seq,
+            "speedup": t_seq / t_batch, "batched_s": t_batch, "one_at_a_time_s": t_seq,
+            "field_steps_per_s_batched": (m + 1) * steps / t_batch, "bitwise_same_as_alone": bool(same)}
+
+
 def pad_spectrum(u, M):
     """The same physical field on an M^3 grid (M > N): the N^3 half spectrum
     zero-padded and scaled by (M/N)^3 (the inverse transform carries 1/M^3);
@@ -330,6 +377,9 @@ def main():
         extra["scalar"] = dict(sc, workload=f"{N}^3 FP64 velocity + WENO5 passive scalar "
                                             "(E_theta ~ k^2 exp(-(k/4)^2), unit variance)")
         print(f"[bench] fp64 + scalar: {sc['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+        extra["fd_jv_batch"] = run_fd_jv_batch(seed=args.seed)
+        print(f"[bench] batched FD J.v: {extra['fd_jv_batch']['jv_per_s_batched']:.1f} J.v/s "
+              f"({extra['fd_jv_batch']['speedup']:.2f}x one at a time)", file=sys.stderr, flush=True)
         if N == 256:
             M = 512
             u512 = pad_spectrum(u0, M)
@@ -392,6 +442,7 @@ def main():
         out["copy_peak"] = dict(extra["copy_peak"], step_frac_of_copy=prim["step_GBps"] / cp,
                                 dominant_frac_of_copy=dom["GBps"] / cp)
         out["scalar"] = extra["scalar"]
+        out["fd_jv_batch"] = extra["fd_jv_batch"]
         if "n512" in extra:
             out["n512"] = extra["n512"]
     for sec in secs:

@@ -141,6 +141,17 @@ int ns3d_weno_advection(ns3d_ctx *ctx, int scheme, const ns3d_cplx *ux, const ns
 int ns3d_rk3_ssp_graph(ns3d_ctx *ctx, int nsteps, ns3d_cplx *ux, ns3d_cplx *uy, ns3d_cplx *uz,
                        const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz);
 
+/* nsteps of RK3_SSP for nbatch independent fields of this context's grid, advanced by one set
+ * of launches (the batch is the grid's z dimension).  u holds the members back to back in the
+ * reference layout: member b's component k is u + (3b + k)*Nx*Ny*Mz.  The forcing (may be null)
+ * is shared.  Each member's result is bitwise what ns3d_rk3_ssp_steps gives it alone (same
+ * kernels, same order).  NS3D_FP32_STATE runs the members one after another.  No reference
+ * counterpart: the reference advances one field per RK3_SSP call (RK_time_step.cu:310-327);
+ * this serves its batched callers (many finite-difference J.v perturbations, parameter sweeps). */
+#define NS3D_MAX_BATCH 1024
+int ns3d_rk3_ssp_batch(ns3d_ctx *ctx, int nbatch, int nsteps, ns3d_cplx *u,
+                       const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz);
+
 /* RK3_SSP of the velocity plus a passive scalar theta (physical, double [Nx][Ny][Nz]) advected
  * with WENO (scheme 3/5) by each stage's 2/3-masked physical velocity, with the same SSP
  * weights: theta_t = -(u.grad) theta.  No reference counterpart as a whole: it couples

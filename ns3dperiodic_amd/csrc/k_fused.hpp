@@ -35,6 +35,9 @@ template <int N> struct XYSel<N, 4> { using type = ZPlan<N>; };
 template <int N, int E> using XYPlan = typename XYSel<N, E>::type;
 // The inverse passes and the y-forward pass handle one field per block
 // (blockIdx.y): shorter blocks, more of them in flight.
+// Batches: blockIdx.z is the member of a batch of same-size fields advanced
+// by one launch; each kernel shifts its per-member pointers by z times the
+// member stride it is given (bsc: scratch, bwi: input state; 0 when z is 1).
 #define NS3D_FIELD_LOOP(f, n) for (int f = blockIdx.y; f < (int)blockIdx.y + 1; f++)
 
 // ---------------------------------------------------------------- P1
@@ -46,7 +49,7 @@ template <int N, int E> using XYPlan = typename XYSel<N, E>::type;
 template <typename TB, int NX, int P, int NF, int E, int OS, typename TS = double>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
 k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int Mw, int Ms,
-             int nlt, int nb, SpecTables tab, const cplx<TB>* __restrict__ twg) {
+             int nlt, int nb, SpecTables tab, const cplx<TB>* __restrict__ twg, long bwi = 0, long bsc = 0) {
     constexpr int TPP = XYPlan<NX, E>::TPP;
     using F = PencilFFT<TB, NX, TPP, +1, typename XYPlan<NX, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NX * P];
@@ -67,7 +70,8 @@ k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc
     const long baseo = (long)k * Ms + l;
     const int f = OS ? (int)blockIdx.y >> 1 : (int)blockIdx.y;
     const int which = OS ? (int)blockIdx.y & 1 : 0;  // OS: 0 -> w_b, 1 -> dx w_b
-    const cplx<TS>* __restrict__ src = w.p[f];
+    const cplx<TS>* __restrict__ src = w.p[f] + (long)blockIdx.z * bwi;
+    A += (long)blockIdx.z * bsc;
     cplx<TB> v[F::E], d[OS ? 1 : F::E];
 #pragma unroll
     for (int q = 0; q < F::B0; q++) {
@@ -115,7 +119,7 @@ k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc
 template <typename TB, int NY, int P, int NF, int E, int OS>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NY, E>::TPP))
 k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, int Ny, int Mz, int Ms, int nlt,
-             int nb, SpecTables tab, const cplx<TB>* __restrict__ twg) {
+             int nb, SpecTables tab, const cplx<TB>* __restrict__ twg, long bsc = 0) {
     constexpr int TPP = XYPlan<NY, E>::TPP;
     using F = PencilFFT<TB, NY, TPP, +1, typename XYPlan<NY, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NY * P];
@@ -125,6 +129,8 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
     const int tid = threadIdx.x, p = tid % P, t = tid / P;
     const int j = bid / nlt, lt = bid % nlt;
     if (lt * P > tab.lmax) return;  // z-planes beyond lmax are zero; the z pass skips them
+    A += (long)blockIdx.z * bsc;
+    B += (long)blockIdx.z * bsc;
     const int l = lt * P + p;
     const bool lv = l < Mz && l <= tab.lmax;
     stage_twiddles<TB, NY>(tw, twg);
@@ -388,7 +394,7 @@ struct ZPencil {
 template <typename TB, int NZ, int P, int NF>
 __global__ void __attribute__((amdgpu_waves_per_eu(NF == 1 ? NS3D_WAVES_Z : 2, 8))) __launch_bounds__((P * ZPlanT<TB, NZ>::TPP))
 k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, long npen, int Mz, int nb,
-            SpecTables tab, const cplx<TB>* __restrict__ twg, TB scale, TB* __restrict__ uphys, long nr) {
+            SpecTables tab, const cplx<TB>* __restrict__ twg, TB scale, TB* __restrict__ uphys, long nr, long bsc = 0) {
     // B is zero on z-planes l > lmax (never written by the y pass): TRUNC skips them
     using Z = ZPencil<TB, NZ, P, true>;
     constexpr int TPP = Z::TPP, NPT = Z::NPT;
@@ -401,6 +407,8 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
     const long pen = (long)bid * P + p;
     const bool valid = pen < npen;
     const long off = valid ? pen * Mz : 0;
+    Bf += (long)blockIdx.z * bsc;
+    C += (long)blockIdx.z * bsc;
     auto fld = [&](int f) { return Bf + (long)f * nc + off; };
     // field ids: s*9 + {0..2: w_b, 3..5: dx w_b, 6..8: dy w_b}
     TB Q[3][NPT];
@@ -529,7 +537,7 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
 template <typename TB, int NA, int P, int DIR, int AXIS, int E>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NA, E>::TPP))
 k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, int Ms, int nlt, int nb,
-           const cplx<TB>* __restrict__ twg) {
+           const cplx<TB>* __restrict__ twg, long bsc = 0) {
     constexpr int TPP = XYPlan<NA, E>::TPP;
     using F = PencilFFT<TB, NA, TPP, DIR, typename XYPlan<NA, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NA * P];
@@ -543,6 +551,8 @@ k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, i
     const bool lv = l < Mz;
     const long S = AXIS == 0 ? (long)Ny * Ms : (long)Ms;
     const long base = AXIS == 0 ? (long)o * Ms + l : (long)o * Ny * Ms + l;
+    in += (long)blockIdx.z * bsc;
+    out += (long)blockIdx.z * bsc;
 #pragma unroll 1
     NS3D_FIELD_LOOP(f, nf) {
         const cplx<TB>* src = in + (long)f * nc;
@@ -585,6 +595,8 @@ struct EpiArgs {
     // mode (j,k,l) of w / u0 / out / f sits at (j*ny + k)*m + l: m = Mz for the
     // reference layout, a padded row stride for the library's own stage buffers
     int ny, mw, mu0, mo, mf;
+    // batch member strides (elements) of w, u0 and out; f is shared by the members
+    long bw, bu0, bo;
 };
 
 // Projection/diffusion/update at one mode (Source/diffusion.cu:96-365).
@@ -680,7 +692,7 @@ __device__ __forceinline__ void ipq_rule(cplx<double>& Qx, cplx<double>& Qy, cpl
 template <typename TB, int NX, int P, int MODE, int E>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
 k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int nlt, int nb, SpecTables tab,
-            const cplx<TB>* __restrict__ twg, EpiArgs ea) {
+            const cplx<TB>* __restrict__ twg, EpiArgs ea, long bsc = 0) {
     constexpr int TPP = XYPlan<NX, E>::TPP;
     using F = PencilFFT<TB, NX, TPP, -1, typename XYPlan<NX, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NX * P];
@@ -694,6 +706,16 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int
     const bool lv = l < Mz;
     const long Si = (long)Ny * Ms;       // D: rows padded to Ms
     const long basei = (long)k * Ms + l;
+    if (blockIdx.z) {                    // batch member z
+        const long z = blockIdx.z;
+        D += z * bsc;
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            ea.w[b] += z * ea.bw;
+            if (ea.u0[b]) ea.u0[b] += z * ea.bu0;
+            ea.out[b] += z * ea.bo;
+        }
+    }
     cplx<TB> qv[3][F::E];
     // the three components are all live until the epilogue: load them together
 #pragma unroll

@@ -176,6 +176,12 @@ struct ns3d_ctx {
     void* uphys = nullptr;
     void* uphys_out = nullptr;  // non-null only while a scalar step wants it
     double* theta_stage = nullptr;  // 2 real FP64 fields
+    // batches of same-size fields advanced by one set of launches (grid.z = member):
+    // members in flight, scratch stride per member (elements), stage buffers per member
+    int batch = 1;
+    long bsc = 0;
+    zd* bstage = nullptr;  // batch members' u1[3], u2[3]: 6 fields of nc8 each per member
+    int bstage_members = 0;
     // optional per-pass timing with HIP events on the context stream
     bool prof_on = false;
     std::vector<hipEvent_t> ev_pool;
@@ -375,16 +381,21 @@ int ensure_fp64(ns3d_ctx* c, zd** p, int fields) {
 
 inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
 // grid for the XCD-remapped pencil kernels: block count padded to a multiple of 8
-inline dim3 xgrid(long nb, int nfy = 1) { return dim3((unsigned)(((nb + 7) / 8) * 8), (unsigned)nfy); }
+inline dim3 xgrid(long nb, int nfy = 1, int nz = 1) {
+    return dim3((unsigned)(((nb + 7) / 8) * 8), (unsigned)nfy, (unsigned)nz);
+}
 
 // ---------------------------------------------------------------- fused pipeline
 // Runs P1..P4 for NF vector fields and leaves D (3 fields) at scratch + 3*nci.
 // mw: row stride of the input fields (0: Mz, the reference layout)
+// bin: batch member stride of the inputs (c->batch members, scratch c->bsc apart).
 template <typename TB, typename TS = double>
-int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw = 0) {
+int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw = 0, long bin = 0) {
     if (mw == 0) mw = c->Mz;
-    int st = ensure_scratch(c, 15 * nf);
+    int st = ensure_scratch(c, 15 * nf * c->batch);
     if (st) return st;
+    const int nz = c->batch;
+    const long bsc = c->bsc;
     // scratch fields: rows padded to Mi (128-B aligned pencil tiles), nci modes each
     const long nc = c->nci;
     const int Mi = mi_of(c);
@@ -400,15 +411,15 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Ny * nlt;
         constexpr int OS = NS3D_OS_XINV;
-        dim3 grid = xgrid(nb, 3 * nf * (OS ? 2 : 1)), blk(P * XYPlan<NX, E>::TPP);
+        dim3 grid = xgrid(nb, 3 * nf * (OS ? 2 : 1), nz), blk(P * XYPlan<NX, E>::TPP);
         if (nf == 1) {
             PtrArr<const cplx<TS>, 3> w{{in[0], in[1], in[2]}};
             hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E, OS, TS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, mw, Mi, nlt, nb, c->tab,
-                               twp<TB>(c, 0));
+                               twp<TB>(c, 0), bin, bsc);
         } else if constexpr (std::is_same<TS, double>::value) {
             PtrArr<const zd, 6> w{{in[0], in[1], in[2], in[3], in[4], in[5]}};
             hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2, E, OS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, mw, Mi, nlt, nb, c->tab,
-                               twp<TB>(c, 0));
+                               twp<TB>(c, 0), bin, bsc);
         } else {
             return NS3D_ERR_ARG;
         }
@@ -425,13 +436,13 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Nx * nlt;
         constexpr int OS = NS3D_OS_YINV;
-        dim3 grid = xgrid(nb, 3 * nf * (OS ? 3 : 1)), blk(P * XYPlan<NY, E>::TPP);
+        dim3 grid = xgrid(nb, 3 * nf * (OS ? 3 : 1), nz), blk(P * XYPlan<NY, E>::TPP);
         if (nf == 1)
             hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 1, E, OS>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, Mi, nlt, nb, c->tab,
-                               twp<TB>(c, 1));
+                               twp<TB>(c, 1), bsc);
         else
             hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 2, E, OS>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, Mi, nlt, nb, c->tab,
-                               twp<TB>(c, 1));
+                               twp<TB>(c, 1), bsc);
         return NS3D_OK;
     });
     prof_end(c);
@@ -442,15 +453,15 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
         constexpr int NZ = decltype(NZc)::value;
         constexpr int P = PZ<TB, NZ>;
         const int nb = (int)nblk(c->npen, P);
-        dim3 grid = xgrid(nb), blk(P * ZPlanT<TB, NZ>::TPP);
+        dim3 grid = xgrid(nb, 1, nz), blk(P * ZPlanT<TB, NZ>::TPP);
         const TB scale = (TB)(1.0 / (1.0 * c->Nx * c->Ny * c->Nz));
         TB* up = nf == 1 ? (TB*)c->uphys_out : nullptr;
         if (nf == 1)
             hipLaunchKernelGGL((k_z_product<TB, NZ, P, 1>), grid, blk, 0, s, B, A, nc, c->npen, Mi, nb, c->tab, twp<TB>(c, 2), scale,
-                               up, c->nr);
+                               up, c->nr, bsc);
         else
             hipLaunchKernelGGL((k_z_product<TB, NZ, P, 2>), grid, blk, 0, s, B, A, nc, c->npen, Mi, nb, c->tab, twp<TB>(c, 2), scale,
-                               (TB*)nullptr, c->nr);
+                               (TB*)nullptr, c->nr, bsc);
         return NS3D_OK;
     });
     prof_end(c);
@@ -463,9 +474,9 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
         constexpr int P = PXY_T<TB, NY, E, NS3D_PYFWD32, 8>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Nx * nlt;
-        dim3 grid = xgrid(nb, 3), blk(P * XYPlan<NY, E>::TPP);
+        dim3 grid = xgrid(nb, 3, nz), blk(P * XYPlan<NY, E>::TPP);
         hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1, E>), grid, blk, 0, s, A, A + 3 * nc, 3, nc, c->Ny, c->Mz, Mi, nlt, nb,
-                           twp<TB>(c, 1));
+                           twp<TB>(c, 1), bsc);
         return NS3D_OK;
     });
     prof_end(c);
@@ -482,10 +493,9 @@ int run_x_forward_epilogue(ns3d_ctx* c, const EpiArgs& ea) {
         constexpr int P = PXY_T<TB, NX, E, NS3D_PXFWD32, 8>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Ny * nlt;
-        dim3 grid = xgrid(nb), blk(P * XYPlan<NX, E>::TPP);
+        dim3 grid = xgrid(nb, 1, c->batch), blk(P * XYPlan<NX, E>::TPP);
         hipLaunchKernelGGL((k_x_fwd_epi<TB, NX, P, MODE, E>), grid, blk, 0, c->stream, D, c->nci, c->Ny, c->Mz, mi_of(c), nlt, nb,
-                           c->tab,
-                           twp<TB>(c, 0), ea);
+                           c->tab, twp<TB>(c, 0), ea, c->bsc);
         return NS3D_OK;
     });
     prof_end(c);
@@ -510,11 +520,11 @@ template <int MODE>
 int fused(ns3d_ctx* c, const zd* const* in, int nf, const EpiArgs& ea) {
     int st;
     if (c->precision == NS3D_FP64) {
-        st = run_nonlinear_passes<double>(c, in, nf, ea.mw);
+        st = run_nonlinear_passes<double>(c, in, nf, ea.mw, ea.bw);
         if (st) return st;
         st = run_x_forward_epilogue<double, MODE>(c, ea);
     } else {
-        st = run_nonlinear_passes<float>(c, in, nf, ea.mw);
+        st = run_nonlinear_passes<float>(c, in, nf, ea.mw, ea.bw);
         if (st) return st;
         st = run_x_forward_epilogue<float, MODE>(c, ea);
     }
@@ -523,12 +533,16 @@ int fused(ns3d_ctx* c, const zd* const* in, int nf, const EpiArgs& ea) {
 }
 
 // E(w) stage: out = a0*u0 + a1*(w + dt*L(w)); mw/mu0/mo: row strides (0: Mz)
+// bw, bu0, bo: batch member strides of w, u0 and out (when c->batch > 1)
 int stage_into(ns3d_ctx* c, const zd* const w[3], const zd* const u0[3], double a0, double a1, zd* const out[3],
-               const zd* const f[3], int mw = 0, int mu0 = 0, int mo = 0) {
+               const zd* const f[3], int mw = 0, int mu0 = 0, int mo = 0, long bw = 0, long bu0 = 0, long bo = 0) {
     EpiArgs ea = make_epi(c);
     if (mw) ea.mw = mw;
     if (mu0) ea.mu0 = mu0;
     if (mo) ea.mo = mo;
+    ea.bw = bw;
+    ea.bu0 = bu0;
+    ea.bo = bo;
     for (int b = 0; b < 3; b++) {
         ea.w[b] = (zd*)w[b];
         ea.u0[b] = u0 ? u0[b] : nullptr;
@@ -606,6 +620,43 @@ int ensure_state32(ns3d_ctx* c) {
 }
 
 // nsteps RK3 steps in the context's precision
+// One RK3 step of nb same-size fields with one set of launches: member b's
+// component k at u + (3b + k)*nc (reference layout); forcing shared.  The
+// kernels and their order are those of rk3_one, so each member's result is
+// bitwise the one rk3_one gives it alone.
+int rk3_one_batch(ns3d_ctx* c, int nb, zd* u, const zd* const f[3]) {
+    const long n8 = c->nc8, nc = c->nc;
+    if (c->bstage_members < nb) {
+        drop_graph(c);
+        if (c->bstage) {
+            hipStreamSynchronize(c->stream);
+            hipFree(c->bstage);
+            c->bstage = nullptr;
+            c->bstage_members = 0;
+        }
+        if (hipMalloc((void**)&c->bstage, sizeof(zd) * (size_t)n8 * 6 * nb) != hipSuccess) return NS3D_ERR_ALLOC;
+        c->bstage_members = nb;
+    }
+    int st = ensure_scratch(c, 15 * nb);
+    if (st) return st;
+    zd* u1[3] = {c->bstage, c->bstage + n8, c->bstage + 2 * n8};
+    zd* u2[3] = {c->bstage + 3 * n8, c->bstage + 4 * n8, c->bstage + 5 * n8};
+    zd* uu[3] = {u, u + nc, u + 2 * nc};
+    const zd* uc[3] = {uu[0], uu[1], uu[2]};
+    const zd* u1c[3] = {u1[0], u1[1], u1[2]};
+    const zd* u2c[3] = {u2[0], u2[1], u2[2]};
+    const int mz = c->Mz, m8 = c->m8;
+    const long bs = 3 * nc, bst = 6 * n8;
+    c->batch = nb;
+    c->bsc = 15 * c->nci;
+    if (!(st = stage_into(c, uc, nullptr, 0.0, 1.0, u1, f, mz, mz, m8, bs, 0, bst)))
+        if (!(st = stage_into(c, u1c, uc, 3.0 / 4.0, 1.0 / 4.0, u2, f, m8, mz, m8, bst, bs, bst)))
+            st = stage_into(c, u2c, uc, 1.0 / 3.0, 2.0 / 3.0, uu, f, m8, mz, mz, bst, bs, bs);
+    c->batch = 1;
+    c->bsc = 0;
+    return st;
+}
+
 int rk3_steps(ns3d_ctx* c, int nsteps, zd* const u[3], const zd* const f[3]) {
     if (c->precision != NS3D_FP32_STATE) {
         for (int i = 0; i < nsteps; i++) {
@@ -903,6 +954,7 @@ int ns3d_destroy(ns3d_ctx* c) {
     }
     hipFree(c->scratch);
     hipFree(c->stage);
+    hipFree(c->bstage);
     hipFree(c->s32);
     hipFree(c->extra);
     hipFree(c->red);
@@ -1130,6 +1182,28 @@ int ns3d_rk3_ssp_steps(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns
     zd* u[3] = {(zd*)ux, (zd*)uy, (zd*)uz};
     const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
     return rk3_steps(c, nsteps, u, f);
+}
+
+int ns3d_rk3_ssp_batch(ns3d_ctx* c, int nbatch, int nsteps, ns3d_cplx* u, const ns3d_cplx* fx, const ns3d_cplx* fy,
+                       const ns3d_cplx* fz) {
+    if (!c || !u || nbatch < 1 || nsteps < 0) return NS3D_ERR_ARG;
+    if (nbatch > NS3D_MAX_BATCH) return NS3D_ERR_SIZE;
+    const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
+    zd* ub = (zd*)u;
+    if (c->precision == NS3D_FP32_STATE) {
+        // the FP32 state loop keeps one member's state: members run one after another
+        for (int b = 0; b < nbatch; b++) {
+            zd* m[3] = {ub + 3 * b * c->nc, ub + (3 * b + 1) * c->nc, ub + (3 * b + 2) * c->nc};
+            int st = rk3_steps(c, nsteps, m, f);
+            if (st) return st;
+        }
+        return NS3D_OK;
+    }
+    for (int i = 0; i < nsteps; i++) {
+        int st = rk3_one_batch(c, nbatch, ub, f);
+        if (st) return st;
+    }
+    return NS3D_OK;
 }
 
 int ns3d_rk3_ssp_scalar(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns3d_cplx* uz, const ns3d_cplx* fx,

@@ -15,6 +15,10 @@ BiCGStab(L) and damped Newton (the Source/Arnoldi stack).
   BiCGStab(L) of Sleijpen and Fokkema (ETNA 1, 1993).
 * ``newton`` -- Newton (Source/Arnoldi/Newton.cu:9-98): x += 0.8 delta with
   J(x) delta = -F(x) solved by BiCGStab(L), until ||F(x)|| < tol.
+* ``fd_jv_batch`` -- finite-difference J.v of the time-T map for many
+  perturbations at once (build_Jacobian's difference quotient,
+  Source/Jacobian.cu:230-318, applied to the RK3 flow map), all members
+  advanced together by ns3d_rk3_ssp_batch.
 
 Vectors live in FP64 on the GPU; every inner product, projection and basis
 update runs in the library's deterministic FP64 kernels (``GpuOps``: the
@@ -480,6 +484,32 @@ def newton(ops, rhs, jacobian, x, tol=1e-10, max_iter=50, alpha=0.8, L=6, lin_to
         if not math.isfinite(fn):
             break
     return {"converged": False, "iterations": it, "residual": fn, "history": hist}
+
+
+# ----------------------------------------------------------------------------- batched FD J.v
+
+
+def fd_jv_batch(g, u, V, eps=1e-7, steps=1, f=None, central=False):
+    """Finite-difference J.v of the steps-step RK3 map Phi at u for every
+    perturbation in V ([m, 3, Nx, Ny, Mz] complex128 on the GPU):
+    forward (Phi(u + eps v) - Phi(u)) / eps, all m + 1 fields in one batch,
+    or central (Phi(u + eps v) - Phi(u - eps v)) / (2 eps) (2m fields), the
+    difference quotient of build_Jacobian (Source/Jacobian.cu:230-318).
+    u: three spectral fields; returns [m, 3, Nx, Ny, Mz]."""
+    import torch
+    u3 = torch.stack([c.to(torch.complex128) for c in u])
+    m = V.shape[0]
+    if central:
+        ub = torch.empty((2 * m,) + tuple(u3.shape), dtype=torch.complex128, device=u3.device)
+        ub[0::2] = u3 + eps * V
+        ub[1::2] = u3 - eps * V
+        g.RK3_SSP_batch(ub, f, steps=steps)
+        return (ub[0::2] - ub[1::2]) / (2.0 * eps)
+    ub = torch.empty((m + 1,) + tuple(u3.shape), dtype=torch.complex128, device=u3.device)
+    ub[0] = u3
+    ub[1:] = u3 + eps * V
+    g.RK3_SSP_batch(ub, f, steps=steps)
+    return (ub[1:] - ub[0]) / eps
 
 
 # ----------------------------------------------------------------------------- NS operators

@@ -190,6 +190,14 @@ class NS3D:
         check(self.lib.ns3d_rk3_ssp_graph(self.ctx, steps, *self._c3(u), *self._f3(f)), "ns3d_rk3_ssp_graph")
         return u
 
+    def RK3_SSP_batch(self, ub, f=None, steps=1):
+        """steps RK3 steps of a batch of fields advanced together: ub is a
+        complex128 tensor [B, 3, Nx, Ny, Mz] (member b, component k)."""
+        assert ub.dtype == torch.complex128 and ub.is_contiguous() and tuple(ub.shape[1:]) == (3,) + self.cshape
+        check(self.lib.ns3d_rk3_ssp_batch(self.ctx, int(ub.shape[0]), steps, _ptr(ub), *self._f3(f)),
+              "ns3d_rk3_ssp_batch")
+        return ub
+
     def RK3_SSP_scalar(self, u, theta, f=None, steps=1, scheme=5):
         """RK3 of the velocity plus a WENO-advected passive scalar theta (physical float64)."""
         assert theta.dtype == torch.float64 and tuple(theta.shape) == self.rshape
