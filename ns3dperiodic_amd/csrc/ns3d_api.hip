@@ -11,6 +11,11 @@
 #include <vector>
 
 #include "../../include/ns3d.h"
+// tile shapes and plans chosen by ns3dperiodic_amd/autotune.py; the #ifndef
+// defaults in the kernel headers apply to whatever it does not set
+#if !defined(NS3D_NO_TUNED_HEADER) && __has_include("tuned.hpp")
+#include "tuned.hpp"
+#endif
 #include "k_misc.hpp"
 #include "k_krylov.hpp"
 #include "k_diag.hpp"
