@@ -150,7 +150,7 @@ def run_scalar(N, steps, warmup, u0, theta0, Re, dt):
 def run_fd_jv_batch(N=64, m=64, Re=150.0, T=1.0, eps=1e-7, seed=1):
     """Batched Krylov work: finite-difference J.v of the time-T map for m
     seeded unit-norm solenoidal perturbations of an N^3 Kolmogorov-forced
-    field (f_x = sin 4y, Re = 150, config-2 spectrum), all m + 1 fields
+    field (f_x = sin 4y, Re = 150, k^4 exp(-2(k/4)^2) spectrum), all m + 1 fields
     advanced together (ns3d_rk3_ssp_batch), against the same m + 1 solves
     one field at a time.  FP64 (eps = 1e-7 is below FP32 resolution)."""
     import torch

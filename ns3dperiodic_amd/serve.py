@@ -103,7 +103,7 @@ def poisson_stream(n, rate, seed=0, sizes=GRID_SIZES, re_range=(50.0, 500.0), t_
 
 
 def request_inputs(req, energy=0.5):
-    """Host inputs of a request: config-2 style seeded solenoidal field
+    """Host inputs of a request: a seeded solenoidal field
     (E(k) ~ k^4 exp(-2 (k/4)^2), energy 0.5), the forcing f_x = sin 4y, and for
     J.v a unit-norm solenoidal perturbation.  Fills req.nsteps / req.dt."""
     N = req.N

@@ -1,4 +1,4 @@
-// Config 1 (Shapiro Beltrami flow) driven from host C++ through the
+// The Shapiro Beltrami flow (32^3, Re = 20) driven from host C++ through the
 // reference-named wrappers of ns3d_compat.hpp: the drop-in as a C++ caller of
 // the reference would use it (device arrays from the HIP runtime, the
 // reference's FFTN_Device / RK3_SSP / iFFTN_Device calls).

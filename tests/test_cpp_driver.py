@@ -25,7 +25,7 @@ def test_driver_compiles(tmp_path):
 
 
 @pytest.mark.gpu
-def test_driver_runs_config1(tmp_path):
+def test_driver_runs_shapiro_32(tmp_path):
     from ns3dperiodic_amd import NS3D, fields
     import torch
     print("\n[case] C++ driver, Shapiro 32^3", flush=True)

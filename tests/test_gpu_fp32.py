@@ -2,7 +2,7 @@
 
 fp32: FP32 transforms and products, FP64 state.  fp32s: FP32 state inside the RK3 loop too.
 
-Config 3 workload: 256^3 decaying turbulence, nu = 2e-3, the seeded k^4 exp(-2(k/4)^2)
+Workload: 256^3 decaying turbulence, nu = 2e-3, the seeded k^4 exp(-2(k/4)^2)
 spectrum, 1000 RK3 steps in each precision from identical initial bits.  Measured on
 MI355X: 1.3e-6 (energy agrees to ~4e-8, dissipation to ~3e-7).
 """

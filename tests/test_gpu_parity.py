@@ -223,8 +223,8 @@ def test_krylov_packing_and_jv(pair):
     assert rel_l2(out, o.jv_reduced(u, vec)) < TOL
 
 
-def test_shapiro_config1():
-    """Config 1: Shapiro Beltrami flow, N=32, nu=0.05, dt=1e-3, 100 RK3 steps."""
+def test_shapiro_32():
+    """Shapiro Beltrami flow, N=32, nu=0.05, dt=1e-3, 100 RK3 steps."""
     from tests.gpu_util import to_dev, to_host
     from ns3dperiodic_amd import NS3D
     g = NS3D(32, Re=20.0, dt=1e-3)
@@ -284,7 +284,7 @@ def test_large_axes(shape):
 
 @pytest.mark.parametrize("shape", [(32, 32, 32), (32, 16, 48)], ids=lambda s: "x".join(map(str, s)))
 def test_rk3_with_passive_scalar(shape):
-    """Velocity RK3 plus a WENO5-advected scalar (config 3's scalar) against the oracle."""
+    """Velocity RK3 plus a WENO5-advected scalar (the decaying-turbulence passive scalar) against the oracle."""
     from tests.gpu_util import to_dev, to_host
     print(f"\n[case] scalar grid {shape}", flush=True)
     g, o = _make(shape, Re=500.0, dt=2e-3)

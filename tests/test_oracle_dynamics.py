@@ -161,7 +161,7 @@ def test_rk3_constant_forcing_polynomial(oracle_mod):
 
 
 def test_shapiro_exact_solution(oracle_mod):
-    """Config 1 at reduced length: the Beltrami field decays as exp(-3 t/Re)."""
+    """The Shapiro flow at reduced length: the Beltrami field decays as exp(-3 t/Re)."""
     N, Re, dt = 16, 20.0, 1e-3
     o = oracle_mod.Oracle(N, Re=Re, dt=dt)
     u = [o.fft_r2c(c) for c in fields.shapiro(N)]
