@@ -191,6 +191,16 @@ int ns3d_reduce_sum(ns3d_ctx *ctx, long long n, const double *in, double *out_ho
 int ns3d_kinetic_energy(ns3d_ctx *ctx, const double *ux, const double *uy, const double *uz, double *out_host);
 int ns3d_dissipation(ns3d_ctx *ctx, const double *ux, const double *uy, const double *uz, double *out_host);
 int ns3d_cfl_dt(ns3d_ctx *ctx, double cfl, const double *ux, const double *uy, const double *uz, double *dt_host);
+/* Physical variants of the two diagnostics above, exact from the spectral fields (no
+ * transform): 0.5 * integral of |u|^2 over the box, and the integral of 2 S:S
+ * (= |grad u|^2 + (div u)^2; the dissipation rate is this over Re).  The reference's
+ * get_kinetic_energy (math_support.cu:744) integrates only uy, uz over one plane area and
+ * get_dissipation (:777-789) uses central differences (f[j+1]-f[j-1])/dx without the 1/2, which
+ * tends to 4x this integral; those are ns3d_kinetic_energy and ns3d_dissipation. */
+int ns3d_kinetic_energy_spectral(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
+                                 double *out_host);
+int ns3d_dissipation_spectral(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,
+                              double *out_host);
 int ns3d_shapiro_error(ns3d_ctx *ctx, double t, const double *ux, const double *uy, const double *uz,
                        double *err_host);                                          /* Shapiro_test_case: L-inf error */
 

@@ -366,18 +366,43 @@ __device__ __forceinline__ void block_finish(double a, double b, double* __restr
 }
 
 // Pointwise producers feeding the reductions (value at flat physical index id).
-enum RedSrc { SRC_ARRAY = 0, SRC_ENERGY = 1, SRC_DISSIPATION = 2, SRC_CFL = 3, SRC_SHAPIRO = 4 };
+// SRC_SPEC_*: over the nc half-spectrum modes of three spectral fields (a, b,
+// c read as interleaved complex), each mode weighted by its Hermitian
+// multiplicity (1 on the l = 0 and l = Nz/2 planes, else 2).
+enum RedSrc {
+    SRC_ARRAY = 0, SRC_ENERGY = 1, SRC_DISSIPATION = 2, SRC_CFL = 3, SRC_SHAPIRO = 4,
+    SRC_SPEC_ENERGY = 5, SRC_SPEC_DISSIPATION = 6
+};
 struct RedArgs {
     const double* a;
     const double* b;
     const double* c;
     Grid3 G;
     double dx, dy, dz, p0, p1;  // p0: CFL number or time; p1: Re
+    const double* kx;           // wavenumber tables (spectral sources)
+    const double* ky;
+    const double* kz;
+    int Mz;
 };
 
 template <int SRC>
 __device__ __forceinline__ double red_value(const RedArgs& r, long id) {
     if (SRC == SRC_ARRAY) return r.a[id];
+    if (SRC == SRC_SPEC_ENERGY || SRC == SRC_SPEC_DISSIPATION) {
+        const int l = (int)(id % r.Mz);
+        const long t1 = id / r.Mz;
+        const int k = (int)(t1 % r.G.Ny), j = (int)(t1 / r.G.Ny);
+        const double w = (l == 0 || 2 * l == r.G.Nz) ? 1.0 : 2.0;
+        const double ar = r.a[2 * id], ai = r.a[2 * id + 1];
+        const double br = r.b[2 * id], bi = r.b[2 * id + 1];
+        const double cr = r.c[2 * id], ci = r.c[2 * id + 1];
+        const double e = (ar * ar + ai * ai) + (br * br + bi * bi) + (cr * cr + ci * ci);
+        if (SRC == SRC_SPEC_ENERGY) return w * e;
+        // |grad u|^2 + (div u)^2 = 2 S:S integrated: |k|^2 |u^|^2 + |k.u^|^2
+        const double kx = r.kx[j], ky = r.ky[k], kz = r.kz[l];
+        const double dr = kx * ar + ky * br + kz * cr, di = kx * ai + ky * bi + kz * ci;
+        return w * (((kx * kx + ky * ky) + kz * kz) * e + (dr * dr + di * di));
+    }
     const int l = (int)(id % r.G.Nz);
     const long t1 = id / r.G.Nz;
     const int k = (int)(t1 % r.G.Ny), j = (int)(t1 / r.G.Ny);

@@ -825,6 +825,10 @@ RedArgs red_args(const ns3d_ctx* c, const double* a, const double* b, const doub
     r.dz = c->Lz / c->Nz;
     r.p0 = 0.0;
     r.p1 = c->Re;
+    r.kx = c->tab.kx;
+    r.ky = c->tab.ky;
+    r.kz = c->tab.kz;
+    r.Mz = c->Mz;
     return r;
 }
 
@@ -1506,6 +1510,26 @@ int ns3d_dissipation(ns3d_ctx* c, const double* ux, const double* uy, const doub
     if (!c || !ux || !uy || !uz || !out) return NS3D_ERR_ARG;
     RedArgs r = red_args(c, ux, uy, uz);
     return reduce<0, SRC_DISSIPATION>(c, r, c->nr, out);
+}
+// Parseval with the unnormalised forward transform: integral over the box of
+// |u|^2 = (Lx Ly Lz / (Nx Ny Nz)^2) * sum over all modes of |u^|^2.
+int ns3d_kinetic_energy_spectral(ns3d_ctx* c, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz,
+                                 double* out) {
+    if (!c || !ux || !uy || !uz || !out) return NS3D_ERR_ARG;
+    RedArgs r = red_args(c, (const double*)ux, (const double*)uy, (const double*)uz);
+    const int st = reduce<0, SRC_SPEC_ENERGY>(c, r, c->nc, out);
+    const double n3 = (double)c->Nx * c->Ny * c->Nz;
+    if (!st) out[0] *= 0.5 * c->Lx * c->Ly * c->Lz / (n3 * n3);
+    return st;
+}
+int ns3d_dissipation_spectral(ns3d_ctx* c, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d_cplx* uz,
+                              double* out) {
+    if (!c || !ux || !uy || !uz || !out) return NS3D_ERR_ARG;
+    RedArgs r = red_args(c, (const double*)ux, (const double*)uy, (const double*)uz);
+    const int st = reduce<0, SRC_SPEC_DISSIPATION>(c, r, c->nc, out);
+    const double n3 = (double)c->Nx * c->Ny * c->Nz;
+    if (!st) out[0] *= c->Lx * c->Ly * c->Lz / (n3 * n3);
+    return st;
 }
 int ns3d_cfl_dt(ns3d_ctx* c, double cfl, const double* ux, const double* uy, const double* uz, double* dt) {
     if (!c || !ux || !uy || !uz || !dt) return NS3D_ERR_ARG;

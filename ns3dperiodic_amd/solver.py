@@ -310,6 +310,20 @@ class NS3D:
         check(self.lib.ns3d_reduce_sum(self.ctx, x.numel(), _ptr(x), ctypes.byref(out)), "ns3d_reduce_sum")
         return out.value
 
+    def kinetic_energy_spectral(self, u_hat):
+        """0.5 * integral of |u|^2 over the box, exact from the spectral fields."""
+        out = ctypes.c_double()
+        check(self.lib.ns3d_kinetic_energy_spectral(self.ctx, *self._c3(u_hat), ctypes.byref(out)),
+              "ns3d_kinetic_energy_spectral")
+        return out.value
+
+    def dissipation_spectral(self, u_hat):
+        """Integral of 2 S:S over the box (|grad u|^2 + (div u)^2), exact from the spectral fields."""
+        out = ctypes.c_double()
+        check(self.lib.ns3d_dissipation_spectral(self.ctx, *self._c3(u_hat), ctypes.byref(out)),
+              "ns3d_dissipation_spectral")
+        return out.value
+
     def get_kinetic_energy(self, u):
         out = ctypes.c_double()
         check(self.lib.ns3d_kinetic_energy(self.ctx, *self._r3(u), ctypes.byref(out)), "ns3d_kinetic_energy")

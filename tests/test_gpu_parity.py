@@ -202,6 +202,62 @@ def test_reductions_and_diagnostics(pair):
     assert g.Shapiro_test_case(0.3, ud) == pytest.approx(o.shapiro_error(0.3, u), rel=1e-14)
 
 
+def test_spectral_energy_and_dissipation(pair):
+    """Physical diagnostics from the spectral fields against physical-space
+    numpy sums (spectral derivatives), and tied to the reference formulas:
+    with u_x = 0 the reference's transversal energy times dz is the spectral
+    energy, and its central-difference dissipation converges to the spectral
+    one at second order."""
+    from tests.gpu_util import to_dev
+    g, o = pair
+    Nx, Ny, Nz = o.Nx, o.Ny, o.Nz
+    L = 2 * np.pi
+    rng = np.random.default_rng(23)
+    uh = [np.fft.rfftn(rng.standard_normal((Nx, Ny, Nz))) for _ in range(3)]
+    for c in uh:                              # no Nyquist modes: their derivative is not real
+        c[Nx // 2] = 0
+        c[:, Ny // 2] = 0
+        c[:, :, Nz // 2] = 0
+    u = [np.fft.irfftn(c, s=(Nx, Ny, Nz), axes=(0, 1, 2)) for c in uh]
+    dV = L ** 3 / (Nx * Ny * Nz)
+    e_ref = 0.5 * dV * sum(float(np.sum(x * x)) for x in u)
+    kx = np.fft.fftfreq(Nx, 1.0 / Nx)[:, None, None]
+    ky = np.fft.fftfreq(Ny, 1.0 / Ny)[None, :, None]
+    kz = np.arange(Nz // 2 + 1)[None, None, :]
+    ks = [kx, ky, kz]
+    grad = [[np.fft.irfftn(1j * ks[i] * uh[j], s=(Nx, Ny, Nz), axes=(0, 1, 2)) for j in range(3)] for i in range(3)]
+    d_ref = dV * sum(float(np.sum(0.5 * (grad[i][j] + grad[j][i]) ** 2)) for i in range(3) for j in range(3))
+    uhd = to_dev(uh)
+    assert g.kinetic_energy_spectral(uhd) == pytest.approx(e_ref, rel=1e-12)
+    assert g.dissipation_spectral(uhd) == pytest.approx(d_ref, rel=1e-12)
+    # the reference's formulas on the same flow
+    z = [np.zeros_like(uh[0]), uh[1], uh[2]]
+    ez = g.kinetic_energy_spectral(to_dev(z))
+    assert g.get_kinetic_energy(to_dev([np.fft.irfftn(c, s=(Nx, Ny, Nz), axes=(0, 1, 2)) for c in z])) * (L / Nz) == \
+        pytest.approx(ez, rel=1e-12)
+
+
+def test_dissipation_formulas_converge():
+    """The reference's central-difference dissipation tends to 4x the
+    spectral value as (k dx)^2 for a smooth field: its difference quotients
+    (f[j+1] - f[j-1])/dx (Source/math_support.cu:777-785) carry no 1/2, so
+    every squared derivative is four times the derivative squared."""
+    from tests.gpu_util import to_dev
+    from ns3dperiodic_amd import NS3D
+    errs = []
+    for N in (16, 32, 64):
+        g = NS3D(N)
+        X, Y, Z = fields.grid(N, N, N)
+        u = [np.sin(Y) * np.cos(2 * Z), np.sin(2 * Z) * np.cos(X), np.sin(X) * np.cos(Y)]
+        spec = g.dissipation_spectral(to_dev([np.fft.rfftn(c) for c in u]))
+        fd = g.get_dissipation(to_dev(u))
+        errs.append(abs(0.25 * fd - spec) / spec)
+        g.close()
+    print(f"[diag] FD vs spectral dissipation: {errs}", flush=True)
+    assert errs[0] > errs[1] > errs[2] > 0
+    assert 3.5 < errs[1] / errs[2] < 4.5        # second order
+
+
 def test_krylov_packing_and_jv(pair):
     from tests.gpu_util import to_dev, to_host
     import torch
