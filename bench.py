@@ -147,6 +147,29 @@ def run_scalar(N, steps, warmup, u0, theta0, Re, dt):
             "step_roofline_frac": sb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "finite": finite}
 
 
+def run_tangent(N, precision, steps, u0, Re, dt):
+    """Tangent-map steps (RK3_SSP_UV: the base flow advanced together with a
+    perturbation through the bilinear passes, 24 real fields per point in the
+    z pass) per second, beside the plain step rate."""
+    import torch
+    from ns3dperiodic_amd import NS3D, fields
+    g = NS3D(N, Re=Re, dt=dt, precision=precision)
+    u = [torch.from_numpy(c).to("cuda:0") for c in u0]
+    v = [torch.from_numpy(c).to("cuda:0") for c in fields.random_solenoidal(N, seed=77, energy=1e-6)]
+    g.RK3_SSP_UV(u, v)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s = torch.cuda.current_stream()
+    e0.record(s)
+    for _ in range(steps):
+        g.RK3_SSP_UV(u, v)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    g.close()
+    return {"precision": precision, "steps_per_s": 1e3 / ms, "ms_per_step": ms}
+
+
 def run_fd_jv_batch(N=64, m=64, Re=150.0, T=1.0, eps=1e-7, seed=1):
     """Batched Krylov work: finite-difference J.v of the time-T map for m
     seeded unit-norm solenoidal perturbations of an N^3 Kolmogorov-forced
@@ -381,6 +404,12 @@ def main():
         extra["scalar"] = dict(sc, workload=f"{N}^3 FP64 velocity + WENO5 passive scalar "
                                             "(E_theta ~ k^2 exp(-(k/4)^2), unit variance)")
         print(f"[bench] fp64 + scalar: {sc['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+        extra["tangent"] = [dict(run_tangent(N, p, 10, u0, Re, dt), vs_plain_step=None) for p in ("fp64", "fp32")]
+        for t in extra["tangent"]:
+            plain = prim if t["precision"] == args.precision else next(
+                (x for x in secs if x["precision"] == t["precision"]), None)
+            t["vs_plain_step"] = t["ms_per_step"] / plain["ms_per_step"] if plain else None
+            print(f"[bench] tangent map {t['precision']}: {t['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
         extra["fd_jv_batch"] = run_fd_jv_batch(seed=args.seed)
         print(f"[bench] batched FD J.v: {extra['fd_jv_batch']['jv_per_s_batched']:.1f} J.v/s "
               f"({extra['fd_jv_batch']['speedup']:.2f}x one at a time)", file=sys.stderr, flush=True)
@@ -447,6 +476,8 @@ def main():
                                 dominant_frac_of_copy=dom["GBps"] / cp)
         out["scalar"] = extra["scalar"]
         out["fd_jv_batch"] = extra["fd_jv_batch"]
+        out["tangent_map"] = {"workload": f"{N}^3 RK3_SSP_UV steps (base + one perturbation)",
+                              "legs": extra["tangent"]}
         if "n512" in extra:
             out["n512"] = extra["n512"]
     for sec in secs:
