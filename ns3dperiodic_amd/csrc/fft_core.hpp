@@ -232,8 +232,9 @@ struct PencilFFT {
     __device__ __forceinline__ static int out_index(int t, int q, int r) { return t + q * TPP + r * (N / RL); }
 
     // tw[m] = exp(-2*pi*i*m/N); DIR=+1 uses the conjugate.
+    // Butterflies of pass I (twiddles, then the radix-R DFTs) on this thread's values.
     template <int I>
-    __device__ __forceinline__ static void pass(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t, int p) {
+    __device__ __forceinline__ static void butterflies(cplx<T>* v, const cplx<T>* __restrict__ tw, int t) {
         constexpr int R = Info::rad[I];
         constexpr int Ns = Info::prod_before(I);
         constexpr int NB = N / R;
@@ -254,30 +255,66 @@ struct PencilFFT {
                 Radix<R>::template dft<DIR>(v + q * R);
             }
         }
+    }
+    // Stockham exchange after pass I: store in the order pass I+1 reads.
+    template <int I>
+    __device__ __forceinline__ static void exch_write(const cplx<T>* v, cplx<T>* lds, int t, int p) {
+        constexpr int R = Info::rad[I];
+        constexpr int Ns = Info::prod_before(I);
+        constexpr int NB = N / R;
+        constexpr int B = (NB + TPP - 1) / TPP;
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int jj = t + q * TPP;
+            if (NB % TPP == 0 || jj < NB) {
+                const int base = (jj / Ns) * Ns * R + (jj % Ns);
+#pragma unroll
+                for (int r = 0; r < R; r++) lds[Addr::template at<Ns>(p, base + r * Ns)] = v[q * R + r];
+            }
+        }
+    }
+    template <int I>
+    __device__ __forceinline__ static void exch_read(cplx<T>* v, const cplx<T>* lds, int t, int p) {
+        constexpr int Ns = Info::prod_before(I);
+        constexpr int R2 = Info::rad[I + 1];
+        constexpr int NB2 = N / R2;
+        constexpr int B2 = (NB2 + TPP - 1) / TPP;
+#pragma unroll
+        for (int q = 0; q < B2; q++) {
+            const int jj = t + q * TPP;
+            if (NB2 % TPP == 0 || jj < NB2) {
+#pragma unroll
+                for (int r = 0; r < R2; r++) v[q * R2 + r] = lds[Addr::template at<Ns>(p, jj + r * NB2)];
+            }
+        }
+    }
+
+    template <int I>
+    __device__ __forceinline__ static void pass(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t, int p) {
+        butterflies<I>(v, tw, t);
         if constexpr (I + 1 < NP) {
-#pragma unroll
-            for (int q = 0; q < B; q++) {
-                const int jj = t + q * TPP;
-                if (NB % TPP == 0 || jj < NB) {
-                    const int base = (jj / Ns) * Ns * R + (jj % Ns);
-#pragma unroll
-                    for (int r = 0; r < R; r++) lds[Addr::template at<Ns>(p, base + r * Ns)] = v[q * R + r];
-                }
-            }
+            exch_write<I>(v, lds, t, p);
             __syncthreads();
-            constexpr int R2 = Info::rad[I + 1];
-            constexpr int NB2 = N / R2;
-            constexpr int B2 = (NB2 + TPP - 1) / TPP;
-#pragma unroll
-            for (int q = 0; q < B2; q++) {
-                const int jj = t + q * TPP;
-                if (NB2 % TPP == 0 || jj < NB2) {
-#pragma unroll
-                    for (int r = 0; r < R2; r++) v[q * R2 + r] = lds[Addr::template at<Ns>(p, jj + r * NB2)];
-                }
-            }
+            exch_read<I>(v, lds, t, p);
             __syncthreads();
             pass<I + 1>(v, lds, tw, t, p);
+        }
+    }
+    // Two independent transforms in step: one barrier per exchange for both,
+    // and each one's LDS round trip overlaps the other's butterflies.
+    template <int I>
+    __device__ __forceinline__ static void pass2(cplx<T>* va, cplx<T>* vb, cplx<T>* la, cplx<T>* lb,
+                                                 const cplx<T>* __restrict__ tw, int t, int p) {
+        butterflies<I>(va, tw, t);
+        butterflies<I>(vb, tw, t);
+        if constexpr (I + 1 < NP) {
+            exch_write<I>(va, la, t, p);
+            exch_write<I>(vb, lb, t, p);
+            __syncthreads();
+            exch_read<I>(va, la, t, p);
+            exch_read<I>(vb, lb, t, p);
+            __syncthreads();
+            pass2<I + 1>(va, vb, la, lb, tw, t, p);
         }
     }
 
@@ -285,6 +322,11 @@ struct PencilFFT {
     // last-pass output pattern on exit.  All threads of the block must call.
     __device__ __forceinline__ static void run(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t, int p) {
         pass<0>(v, lds, tw, t, p);
+    }
+    // Two transforms at once; la and lb are disjoint exchange buffers of the block's layout.
+    __device__ __forceinline__ static void run2(cplx<T>* va, cplx<T>* vb, cplx<T>* la, cplx<T>* lb,
+                                                const cplx<T>* __restrict__ tw, int t, int p) {
+        pass2<0>(va, vb, la, lb, tw, t, p);
     }
 };
 
