@@ -25,6 +25,13 @@ template <typename T, int N> struct PtrArr { T* p[N]; };
 #ifndef NS3D_WAVES_XY
 #define NS3D_WAVES_XY 2
 #endif
+// Two transforms in step (PencilFFT::run2, a second exchange buffer) for the
+// first two components of the x-forward pass: 0 off, 1 FP64 only, 2 both
+// precisions (measured at 256^3: FP64 pass 0.407 -> 0.401 ms, its register
+// spill gone; the x- and y-inverse passes, at 4-7 waves/SIMD, gained nothing).
+#ifndef NS3D_XFWD_ILP
+#define NS3D_XFWD_ILP 2
+#endif
 #ifndef NS3D_WAVES_Z
 #define NS3D_WAVES_Z 2
 #endif
@@ -794,7 +801,8 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int
             const cplx<TB>* __restrict__ twg, EpiArgs ea, long bsc = 0) {
     constexpr int TPP = XYPlan<NX, E>::TPP;
     using F = PencilFFT<TB, NX, TPP, -1, typename XYPlan<NX, E>::R, IndexMajor<P>>;
-    __shared__ cplx<TB> lds[NX * P];
+    constexpr bool ILP = NS3D_HOIST && (NS3D_XFWD_ILP == 2 || (NS3D_XFWD_ILP == 1 && sizeof(TB) == 8));
+    __shared__ cplx<TB> lds[(ILP ? 2 : 1) * NX * P];
     __shared__ cplx<TB> tw[NX];
     const int bid = xcd_tile();
     if (bid >= nb) return;
@@ -831,7 +839,10 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int
         }
         if (!NS3D_HOIST) F::run(qv[b], lds, tw, t, p);
     }
-    if (NS3D_HOIST) {
+    if constexpr (ILP) {
+        F::run2(qv[0], qv[1], lds, lds + NX * P, tw, t, p);
+        F::run(qv[2], lds, tw, t, p);
+    } else if (NS3D_HOIST) {
 #pragma unroll
         for (int b = 0; b < 3; b++) F::run(qv[b], lds, tw, t, p);
     }
