@@ -1,7 +1,8 @@
 """Autotuner for the tile shapes and FFT plans of the fused RK3 passes.
 
 The kernels take their per-pass shapes as compile-time constants (points per
-thread E, pencils per block P, z-pass plan and block size, LDS swizzle), so
+thread E, pencils per block P, z-pass plan and block size, LDS swizzle, two
+transforms in step), so
 tuning is a build-and-time search over library variants:
 
     python -m ns3dperiodic_amd.autotune build [--out DIR] [-j 6]   # CPU: one variant per candidate
@@ -40,6 +41,8 @@ SPACE = {
     "NS3D_PYINV64": (8, [4, 8], ["fp64"]),
     "NS3D_PZT64": (128, [64, 128, 256], ["fp64"]),
     "NS3D_ZSWZ": (16, [0, 16], ["fp64"]),
+    "NS3D_ZILP": (1, [0, 1], ["fp64"]),
+    "NS3D_XFWD_ILP": (2, [0, 1, 2], ["fp64", "fp32", "fp32s"]),
     "NS3D_EXINV32": (8, [4, 8], ["fp32", "fp32s"]),
     "NS3D_EYINV32": (4, [4, 8], ["fp32", "fp32s"]),
     "NS3D_PXINV32": (8, [4, 8, 16], ["fp32", "fp32s"]),
