@@ -464,9 +464,6 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
     auto fld = [&](int f) { return Bf + (long)f * nc + off; };
     // field ids: s*9 + {0..2: w_b, 3..5: dx w_b, 6..8: dy w_b}
     TB Q[3][NPT];
-#ifndef NS3D_ZPREFETCH
-#define NS3D_ZPREFETCH 1
-#endif
     if constexpr (ILP) {
         // Two paired transforms at a time, the inputs of the next two loaded
         // before the current ones run; the same pairs and products as below.
@@ -517,7 +514,7 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
             Q[1][e] = w[0][e] * g3[e] + w[1][e] * g4[e] + w[2][e] * g0[e];
             Q[2][e] = w[0][e] * Q[2][e] + w[1][e] * g1[e] + w[2][e] * g2[e];
         }
-    } else if constexpr (NF == 1 && NS3D_ZPREFETCH) {
+    } else if constexpr (NF == 1) {
         // Software-pipelined: the loads of pair i+1 are issued before the
         // transform of pair i, so HBM latency overlaps the LDS passes.
         TB w[3][NPT], a[NPT], b[NPT], keep[NPT];
@@ -555,30 +552,6 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[1][e] = w[0][e] * keep[e] + w[1][e] * a[e] + w[2][e] * b[e];
         run_pair(a, b, false, true, nullptr, nullptr);        // dy w_z, dz w_z
-#pragma unroll
-        for (int e = 0; e < NPT; e++) Q[2][e] = w[0][e] * Q[2][e] + w[1][e] * a[e] + w[2][e] * b[e];
-    } else if constexpr (NF == 1) {
-        TB w[3][NPT], a[NPT], b[NPT], keep[NPT];
-        Z::c2r_pair(w[0], w[1], fld(0), false, fld(1), false, valid, tab, lds, tw, t, p, scale);
-        Z::c2r_pair(w[2], keep, fld(2), false, fld(3), false, valid, tab, lds, tw, t, p, scale);  // w_z, dx w_x
-        if (uphys && valid) {
-            // the masked physical velocity, for a scalar advected by this stage's flow
-#pragma unroll
-            for (int e = 0; e < NPT; e++) {
-                const long g = pen * NZ + Z::FI::out_index(t, e / Z::FI::RL, e % Z::FI::RL);
-                uphys[g] = w[0][e];
-                uphys[nr + g] = w[1][e];
-                uphys[2 * nr + g] = w[2][e];
-            }
-        }
-        Z::c2r_pair(a, b, fld(6), false, fld(0), true, valid, tab, lds, tw, t, p, scale);         // dy w_x, dz w_x
-#pragma unroll
-        for (int e = 0; e < NPT; e++) Q[0][e] = w[0][e] * keep[e] + w[1][e] * a[e] + w[2][e] * b[e];
-        Z::c2r_pair(keep, a, fld(4), false, fld(7), false, valid, tab, lds, tw, t, p, scale);     // dx w_y, dy w_y
-        Z::c2r_pair(b, Q[2], fld(1), true, fld(5), false, valid, tab, lds, tw, t, p, scale);      // dz w_y, dx w_z
-#pragma unroll
-        for (int e = 0; e < NPT; e++) Q[1][e] = w[0][e] * keep[e] + w[1][e] * a[e] + w[2][e] * b[e];
-        Z::c2r_pair(a, b, fld(8), false, fld(2), true, valid, tab, lds, tw, t, p, scale);         // dy w_z, dz w_z
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[2][e] = w[0][e] * Q[2][e] + w[1][e] * a[e] + w[2][e] * b[e];
     } else {
