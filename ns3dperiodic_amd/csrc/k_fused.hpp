@@ -253,7 +253,10 @@ struct ZPencil {
 #endif
     // LDS swizzle group in elements (0: none); measured gains: FP64 with the 4-point
     // plan, FP32 with the 8-point plan
-    static constexpr int SWG = sizeof(TB) == 8 ? NS3D_ZSWZ : NS3D_ZSWZ32;
+#ifndef NS3D_R3_ZSWZ
+#define NS3D_R3_ZSWZ 0
+#endif
+    static constexpr int SWG = sizeof(TB) == 8 ? (NZ % 3 == 0 && NZ >= 192 ? NS3D_R3_ZSWZ : NS3D_ZSWZ) : NS3D_ZSWZ32;
     using LA = typename std::conditional<SWG == 0, PencilMajor<NZ, PAD>, PencilMajorSw<NZ, PAD, (SWG == 0 ? 1 : SWG)>>::type;
     using FI = PencilFFT<TB, NZ, TPP, +1, typename ZPlanT<TB, NZ>::R, LA>;
     using FF = PencilFFT<TB, NZ, TPP, -1, typename SeqRev<typename ZPlanT<TB, NZ>::R>::type, LA>;

@@ -120,9 +120,30 @@ template <typename TB> constexpr int E_XFWD = sizeof(TB) == 8 ? NS3D_EXFWD64 : 4
 #ifndef NS3D_PZT64
 #define NS3D_PZT64 128
 #endif
-template <typename TB> constexpr int PZT = sizeof(TB) == 8 ? NS3D_PZT64 : NS3D_PZT32;
+// Radix-3 lengths from 192 up have their own block shapes (NS3D_R3_*, and
+// NS3D_R3_ZSWZ in the z pass), swept at 192^3 (FP64 366 -> 419, FP32 680 ->
+// 700, FP32 state 725 -> 768 steps/s); at 48 and 96 these shapes were slower
+// than the ones above (swept at 256^3), which those lengths keep.
+#ifndef NS3D_R3_PXINV64
+#define NS3D_R3_PXINV64 4
+#endif
+#ifndef NS3D_R3_PYINV64
+#define NS3D_R3_PYINV64 4
+#endif
+#ifndef NS3D_R3_PZT64
+#define NS3D_R3_PZT64 256
+#endif
+#ifndef NS3D_R3_PXFWD32
+#define NS3D_R3_PXFWD32 16
+#endif
+#ifndef NS3D_R3_PYINV32
+#define NS3D_R3_PYINV32 4
+#endif
+#define NS3D_BY_RADIX(N, A, R3) ((N) % 3 == 0 && (N) >= 192 ? (R3) : (A))
 template <typename TB, int N>
-constexpr int PZ = (PZT<TB> / ZPlanT<TB, N>::TPP) > 0 ? (PZT<TB> / ZPlanT<TB, N>::TPP) : 1;
+constexpr int PZT = sizeof(TB) == 8 ? NS3D_BY_RADIX(N, NS3D_PZT64, NS3D_R3_PZT64) : NS3D_PZT32;
+template <typename TB, int N>
+constexpr int PZ = (PZT<TB, N> / ZPlanT<TB, N>::TPP) > 0 ? (PZT<TB, N> / ZPlanT<TB, N>::TPP) : 1;
 
 #define HIPCHK(x)                                         \
     do {                                                  \
@@ -412,7 +433,7 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
     st = with_size(c->Nx, [&](auto NXc) -> int {
         constexpr int NX = decltype(NXc)::value;
         constexpr int E = E_XINV<TB>;
-        constexpr int P = PXY_T<TB, NX, E, NS3D_PXINV32, NS3D_PXINV64>;
+        constexpr int P = PXY_T<TB, NX, E, NS3D_PXINV32, NS3D_BY_RADIX(NX, NS3D_PXINV64, NS3D_R3_PXINV64)>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Ny * nlt;
         constexpr int OS = NS3D_OS_XINV;
@@ -437,7 +458,8 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
     st = with_size(c->Ny, [&](auto NYc) -> int {
         constexpr int NY = decltype(NYc)::value;
         constexpr int E = E_YINV<TB>;
-        constexpr int P = PXY_T<TB, NY, E, NS3D_PYINV32, NS3D_PYINV64>;
+        constexpr int P = PXY_T<TB, NY, E, NS3D_BY_RADIX(NY, NS3D_PYINV32, NS3D_R3_PYINV32),
+                                NS3D_BY_RADIX(NY, NS3D_PYINV64, NS3D_R3_PYINV64)>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Nx * nlt;
         constexpr int OS = NS3D_OS_YINV;
@@ -495,7 +517,7 @@ int run_x_forward_epilogue(ns3d_ctx* c, const EpiArgs& ea) {
     int st = with_size(c->Nx, [&](auto NXc) -> int {
         constexpr int NX = decltype(NXc)::value;
         constexpr int E = E_XFWD<TB>;
-        constexpr int P = PXY_T<TB, NX, E, NS3D_PXFWD32, 8>;
+        constexpr int P = PXY_T<TB, NX, E, NS3D_BY_RADIX(NX, NS3D_PXFWD32, NS3D_R3_PXFWD32), 8>;
         const int nlt = (c->Mz + P - 1) / P;
         const int nb = c->Ny * nlt;
         dim3 grid = xgrid(nb, 1, c->batch), blk(P * XYPlan<NX, E>::TPP);
