@@ -50,6 +50,25 @@ def test_bench_pass_bytes_total_159_fields():
     assert bench.step_bytes(nc, 8, 16) == pytest.approx((159 + 3 * 3 + 3 * 6 + 2 * 3) * 8 * nc)
 
 
+def test_bench_pad_spectrum_keeps_the_field():
+    """The 512^3 leg's initial field is the 256^3 one zero-padded: the same
+    physical field sampled on the finer grid (N/2 planes dropped)."""
+    import numpy as np
+    import bench
+    from ns3dperiodic_amd import fields
+    N, M = 16, 32
+    u = fields.random_solenoidal(N, seed=3)
+    for c in u:
+        c[N // 2] = 0
+        c[:, N // 2] = 0
+        c[:, :, N // 2] = 0
+    v = bench.pad_spectrum(u, M)
+    for a, b in zip(u, v):
+        x = np.fft.irfftn(a, s=(N, N, N), axes=(0, 1, 2))
+        y = np.fft.irfftn(b, s=(M, M, M), axes=(0, 1, 2))
+        assert np.abs(y[::2, ::2, ::2] - x).max() < 1e-14 * np.abs(x).max()
+
+
 def test_profiled_traffic_lookup():
     import bench
     t = bench.profiled_traffic("fp64", "k_z_product")
