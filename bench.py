@@ -147,6 +147,30 @@ def run_scalar(N, steps, warmup, u0, theta0, Re, dt):
             "step_roofline_frac": sb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "finite": finite}
 
 
+def run_accuracy(N, steps, u0, Re, dt, precisions=("fp32", "fp32s")):
+    """Relative L2 distance of each FP32 mode from the FP64 path after `steps`
+    RK3 steps from identical initial bits (the FP32 accuracy bar is 1e-5
+    after 1000 steps)."""
+    import torch
+    from ns3dperiodic_amd import NS3D
+    g = NS3D(N, Re=Re, dt=dt, precision="fp64")
+    ref = [torch.from_numpy(c).to("cuda:0") for c in u0]
+    g.RK3_SSP(ref, None, steps=steps)
+    g.close()
+    den = sum(float(torch.sum(torch.abs(y) ** 2)) for y in ref)
+    out = {"steps": steps}
+    for p in precisions:
+        g = NS3D(N, Re=Re, dt=dt, precision=p)
+        u = [torch.from_numpy(c).to("cuda:0") for c in u0]
+        g.RK3_SSP(u, None, steps=steps)
+        g.close()
+        out[names_of(p)] = math.sqrt(sum(float(torch.sum(torch.abs(x - y) ** 2)) for x, y in zip(u, ref)) / den)
+        del u
+    del ref
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_tangent(N, precision, steps, u0, Re, dt):
     """Tangent-map steps (RK3_SSP_UV: the base flow advanced together with a
     perturbation through the bilinear passes, 24 real fields per point in the
@@ -372,8 +396,11 @@ def main():
     ap.add_argument("--serve-chunk", type=int, default=16, help="RK3 steps per captured graph")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (3 times): the reference kernel for HBM counters")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="RK3 steps after which the FP32 modes are compared with FP64 (0: skip)")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the 512^3 leg, the passive-scalar leg and the copy-peak measurement")
+                    help="only the main timing: skip the FP32 accuracy, 512^3, passive-scalar, tangent-map, "
+                         "batched J.v and copy-peak legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
@@ -404,6 +431,12 @@ def main():
             continue
         secs.append(run_gpu(N, p, args.steps, args.warmup, u0, Re, dt))
         print(f"[bench] {p}: {secs[-1]['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+
+    accuracy = None
+    if args.accuracy_steps > 0 and not args.no_extra:
+        accuracy = run_accuracy(N, args.accuracy_steps, u0, Re, dt)
+        print(f"[bench] rel L2 vs fp64 after {args.accuracy_steps} steps: "
+              f"{ {k: v for k, v in accuracy.items() if k != 'steps'} }", file=sys.stderr, flush=True)
 
     extra = None
     if not args.no_extra:
@@ -479,6 +512,7 @@ def main():
         "passes": prim["passes"],
         "finite": prim["finite"],
         "serving": serving,
+        "fp32_accuracy": accuracy,
     }
     if extra is not None:
         cp = extra["copy_peak"]["GBps"]
