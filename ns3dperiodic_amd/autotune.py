@@ -6,7 +6,7 @@ transforms in step), so
 tuning is a build-and-time search over library variants:
 
     python -m ns3dperiodic_amd.autotune build [--out DIR] [-j 6]   # CPU: one variant per candidate
-    python -m ns3dperiodic_amd.autotune time  [--out DIR] [--N 256]   # GPU: time every variant
+    python -m ns3dperiodic_amd.autotune time  [--out DIR] [--N 256] [--N3 192]   # GPU: time every variant
     python -m ns3dperiodic_amd.autotune select [--out DIR]            # write csrc/tuned.hpp
     python -m ns3dperiodic_amd.build                                  # rebuild with the choice
 
@@ -14,7 +14,8 @@ tuning is a build-and-time search over library variants:
 defaults) and, for every parameter, one variant per alternative value with
 only that parameter changed.  ``time`` runs each variant's RK3 step on the
 GPU (HIP events, steady state) in the precisions the parameter affects and
-records ms/step.  ``select`` keeps, for each parameter, the value with the
+records ms/step: parameters of the power-of-two lengths at N (256), those of
+the radix-3 lengths from 192 up (NS3D_R3_*) at N3 (192).  ``select`` keeps, for each parameter, the value with the
 lowest time when it beats the baseline by more than ``--margin`` (parameters
 of different passes act on different kernels, so their gains add), and
 writes ``csrc/tuned.hpp``, which ns3d_api.hip includes before the defaults.
@@ -32,26 +33,33 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED = os.path.join(HERE, "csrc", "tuned.hpp")
 DEFAULT_OUT = os.path.join(HERE, "_tune")
 
-# parameter -> (source default, candidates, precisions whose step it changes)
+# parameter -> (source default, candidates, precisions whose step it changes,
+#               lengths it applies to: "pow2" timed at N, "r3" (radix-3, >= 192) at N3)
 SPACE = {
-    "NS3D_EXINV64": (4, [4, 8], ["fp64"]),
-    "NS3D_EYINV64": (4, [4, 8], ["fp64"]),
-    "NS3D_EXFWD64": (8, [4, 8], ["fp64"]),
-    "NS3D_PXINV64": (8, [4, 8], ["fp64"]),
-    "NS3D_PYINV64": (8, [4, 8], ["fp64"]),
-    "NS3D_PZT64": (128, [64, 128, 256], ["fp64"]),
-    "NS3D_ZSWZ": (16, [0, 16], ["fp64"]),
-    "NS3D_ZILP": (1, [0, 1], ["fp64"]),
-    "NS3D_XFWD_ILP": (2, [0, 1, 2], ["fp64", "fp32", "fp32s"]),
-    "NS3D_EXINV32": (8, [4, 8], ["fp32", "fp32s"]),
-    "NS3D_EYINV32": (4, [4, 8], ["fp32", "fp32s"]),
-    "NS3D_PXINV32": (8, [4, 8, 16], ["fp32", "fp32s"]),
-    "NS3D_PYINV32": (8, [4, 8, 16], ["fp32", "fp32s"]),
-    "NS3D_PYFWD32": (16, [8, 16], ["fp32", "fp32s"]),
-    "NS3D_PXFWD32": (8, [4, 8, 16], ["fp32", "fp32s"]),
-    "NS3D_PZT32": (128, [64, 128, 256], ["fp32", "fp32s"]),
-    "NS3D_ZPLAN32": (8, [4, 8], ["fp32", "fp32s"]),
-    "NS3D_ZSWZ32": (16, [0, 16], ["fp32", "fp32s"]),
+    "NS3D_EXINV64": (4, [4, 8], ["fp64"], "pow2"),
+    "NS3D_EYINV64": (4, [4, 8], ["fp64"], "pow2"),
+    "NS3D_EXFWD64": (8, [4, 8], ["fp64"], "pow2"),
+    "NS3D_PXINV64": (8, [4, 8], ["fp64"], "pow2"),
+    "NS3D_PYINV64": (8, [4, 8], ["fp64"], "pow2"),
+    "NS3D_PZT64": (128, [64, 128, 256], ["fp64"], "pow2"),
+    "NS3D_ZSWZ": (16, [0, 16], ["fp64"], "pow2"),
+    "NS3D_ZILP": (1, [0, 1], ["fp64"], "pow2"),
+    "NS3D_XFWD_ILP": (2, [0, 1, 2], ["fp64", "fp32", "fp32s"], "pow2"),
+    "NS3D_EXINV32": (8, [4, 8], ["fp32", "fp32s"], "pow2"),
+    "NS3D_EYINV32": (4, [4, 8], ["fp32", "fp32s"], "pow2"),
+    "NS3D_PXINV32": (8, [4, 8, 16], ["fp32", "fp32s"], "pow2"),
+    "NS3D_PYINV32": (8, [4, 8, 16], ["fp32", "fp32s"], "pow2"),
+    "NS3D_PYFWD32": (16, [8, 16], ["fp32", "fp32s"], "pow2"),
+    "NS3D_PXFWD32": (8, [4, 8, 16], ["fp32", "fp32s"], "pow2"),
+    "NS3D_PZT32": (128, [64, 128, 256], ["fp32", "fp32s"], "pow2"),
+    "NS3D_ZPLAN32": (8, [4, 8], ["fp32", "fp32s"], "pow2"),
+    "NS3D_ZSWZ32": (16, [0, 16], ["fp32", "fp32s"], "pow2"),
+    "NS3D_R3_PXINV64": (4, [4, 8], ["fp64"], "r3"),
+    "NS3D_R3_PYINV64": (4, [4, 8], ["fp64"], "r3"),
+    "NS3D_R3_PZT64": (256, [128, 256], ["fp64"], "r3"),
+    "NS3D_R3_ZSWZ": (0, [0, 16], ["fp64"], "r3"),
+    "NS3D_R3_PXFWD32": (16, [8, 16], ["fp32", "fp32s"], "r3"),
+    "NS3D_R3_PYINV32": (4, [4, 8], ["fp32", "fp32s"], "r3"),
 }
 
 
@@ -76,7 +84,7 @@ def variants():
     """(name, overrides) for the baseline and every single-parameter change."""
     base = baseline()
     out = [("base", {})]
-    for k, (_, cands, _) in SPACE.items():
+    for k, (_, cands, _, _) in SPACE.items():
         for c in cands:
             if c != base[k]:
                 out.append((f"{k}={c}", {k: c}))
@@ -138,37 +146,46 @@ def cmd_time(args):
     import numpy as np
     from . import fields
     plan = json.load(open(os.path.join(args.out, "plan.json")))
-    N = args.N
-    u0 = fields.random_solenoidal(N, seed=1, kind="k4exp", k0=4.0, energy=0.5)
-    umax = max(float(np.abs(np.fft.irfftn(c, s=(N, N, N), axes=(0, 1, 2))).max()) for c in u0)
-    dt = float("%.1e" % (0.2 * (2 * np.pi / N) / umax))
+    sizes = {"pow2": args.N, "r3": args.N3}
+    work = {}
+    for cls, N in sizes.items():
+        u0 = fields.random_solenoidal(N, seed=1, kind="k4exp", k0=4.0, energy=0.5)
+        umax = max(float(np.abs(np.fft.irfftn(c, s=(N, N, N), axes=(0, 1, 2))).max()) for c in u0)
+        work[cls] = (N, u0, float("%.1e" % (0.2 * (2 * np.pi / N) / umax)))
     res = {}
     for rep in range(args.reps):
         for v in plan["variants"]:
-            precs = ["fp64", "fp32", "fp32s"] if not v["overrides"] else \
-                SPACE[next(iter(v["overrides"]))][2]
-            for p in precs:
-                ms, ok = _time_lib(os.path.join(args.out, v["lib"]), N, p, args.steps, 3, u0, 500.0, dt)
-                key = f"{v['name']}|{p}"
-                res.setdefault(key, []).append(ms if ok else float("inf"))
-                print(f"[autotune] {v['name']:18s} {p:6s} {ms:.4f} ms/step{'' if ok else ' NON-FINITE'}", flush=True)
+            if v["overrides"]:
+                _, _, precs, cls = SPACE[next(iter(v["overrides"]))]
+                runs = [(cls, precs)]
+            else:   # the baseline at both sizes
+                runs = [(cls, ["fp64", "fp32", "fp32s"]) for cls in sizes]
+            for cls, precs in runs:
+                N, u0, dt = work[cls]
+                for p in precs:
+                    ms, ok = _time_lib(os.path.join(args.out, v["lib"]), N, p, args.steps, 3, u0, 500.0, dt)
+                    res.setdefault(f"{v['name']}@{N}|{p}", []).append(ms if ok else float("inf"))
+                    print(f"[autotune] {v['name']:20s} N={N:3d} {p:6s} {ms:.4f} ms/step{'' if ok else ' NON-FINITE'}",
+                          flush=True)
     import torch
     props = torch.cuda.get_device_properties(0)
-    out = {"N": N, "dt": dt, "steps": args.steps, "gpu": torch.cuda.get_device_name(0),
-           "arch": getattr(props, "gcnArchName", "").split(":")[0],
+    out = {"N": args.N, "N3": args.N3, "dt": {cls: w[2] for cls, w in work.items()}, "steps": args.steps,
+           "gpu": torch.cuda.get_device_name(0), "arch": getattr(props, "gcnArchName", "").split(":")[0],
            "ms_per_step": {k: min(x) for k, x in res.items()}}
-    json.dump(out, open(os.path.join(args.out, f"times_{N}.json"), "w"), indent=1)
+    json.dump(out, open(os.path.join(args.out, "times.json"), "w"), indent=1)
 
 
 def cmd_select(args):
     plan = json.load(open(os.path.join(args.out, "plan.json")))
-    t = json.load(open(os.path.join(args.out, f"times_{args.N}.json")))
+    t = json.load(open(os.path.join(args.out, "times.json")))
     ms = t["ms_per_step"]
     base = plan["baseline"]
     chosen, notes = dict(base), []
-    for k, (_, cands, precs) in SPACE.items():
+    for k, (_, cands, precs, cls) in SPACE.items():
+        N = t["N"] if cls == "pow2" else t["N3"]
+
         def cost(name):
-            return sum(ms[f"{name}|{p}"] for p in precs)
+            return sum(ms[f"{name}@{N}|{p}"] for p in precs)
         best, bcost = base[k], cost("base")
         for c in cands:
             if c == base[k]:
@@ -178,9 +195,10 @@ def cmd_select(args):
                 best, bcost = c, cc
         chosen[k] = best
         notes.append(f"{k} = {best} ({'changed' if best != base[k] else 'kept'}; "
-                     f"{', '.join(precs)} step {bcost:.4f} ms summed)")
+                     f"{', '.join(precs)} step at {N}^3 {bcost:.4f} ms summed)")
     lines = ["// tuned.hpp -- written by ns3dperiodic_amd/autotune.py (select); do not edit by hand.",
-             f"// Timed on {t.get('gpu', 'the GPU')} ({t.get('arch') or 'gfx950'}) at N = {t['N']} (RK3 step, dt = {t['dt']:g}); "
+             f"// Timed on {t.get('gpu', 'the GPU')} ({t.get('arch') or 'gfx950'}): RK3 steps at {t['N']}^3 "
+             f"(power-of-two parameters) and {t['N3']}^3 (NS3D_R3_*); "
              f"a value replaces the baseline only when its step is faster by more than {args.margin:.1%}.",
              "#pragma once"]
     for n in notes:
@@ -196,7 +214,8 @@ def main():
     ap.add_argument("phase", choices=["build", "time", "select"])
     ap.add_argument("--out", default=DEFAULT_OUT)
     ap.add_argument("-j", "--jobs", type=int, default=min(6, os.cpu_count() or 1))
-    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--N", type=int, default=256, help="grid for the power-of-two parameters")
+    ap.add_argument("--N3", type=int, default=192, help="grid for the radix-3 parameters (NS3D_R3_*)")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--margin", type=float, default=0.005)
