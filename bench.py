@@ -299,6 +299,63 @@ def cpu_baseline(N, u0, Re, dt, steps, threads):
                       f"(OpenMP, own mixed-radix FFT, not FFTW), {dt_s:.1f} s"}
 
 
+def run_step_graph(K=20, reps=10, branches=12, seed=1):
+    """Many small fields of uneven sizes advanced by one captured HIP graph
+    (StepGraph: K RK3 steps of every field per replay, fields dealt to
+    ``branches`` concurrent graph branches) against the same contexts stepped
+    eagerly, K steps each, on one stream and spread over ``branches``
+    streams.  FP64, Kolmogorov forcing f_x = 0.1 sin 2y on every field."""
+    import torch
+    from ns3dperiodic_amd import NS3D, StepGraph, fields
+    shapes = [(32, 32, 32)] * 8 + [(48, 64, 32)] * 4 + [(64, 64, 64)] * 4 + [(16, 16, 16)] * 8
+    items, refs = [], []
+    for i, sh in enumerate(shapes):
+        u0 = fields.random_solenoidal(*sh, seed=seed + i, kind="k4exp", k0=4.0, energy=0.5)
+        f = [torch.from_numpy(c).cuda() for c in fields.kolmogorov_forcing_sin(*sh, n=2, amp=0.1)]
+        u = [torch.from_numpy(c).cuda() for c in u0]
+        items.append((NS3D(*sh, Re=200.0, dt=2e-3), u, f))
+        refs.append((NS3D(*sh, Re=200.0, dt=2e-3), [c.clone() for c in u], f))
+    sg = StepGraph(items, steps=K, branches=branches)
+    streams = [torch.cuda.Stream() for _ in range(branches)]
+    # untimed warm-up: one replay, and the same K steps eagerly (states stay in step)
+    sg.replay()
+    for e, r, f in refs:
+        e.set_stream(streams[0])
+        e.RK3_SSP(r, f, steps=K)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sg.replay()
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t0
+    t_eager = {}
+    for ns in (1, branches):
+        for i, (e, _, _) in enumerate(refs):
+            e.set_stream(streams[i % ns])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for e, r, f in refs:
+                e.RK3_SSP(r, f, steps=K)
+        torch.cuda.synchronize()
+        t_eager[ns] = time.perf_counter() - t0
+    # the eager states took two timed legs, the graph's one: replay as many again
+    for _ in range(reps):
+        sg.replay()
+    torch.cuda.synchronize()
+    same = all(torch.equal(a, b) for (_, u, _), (_, r, _) in zip(items, refs) for a, b in zip(u, r))
+    for g, _, _ in items + refs:
+        g.close()
+    n = len(shapes) * K * reps
+    return {"workload": f"{len(shapes)} FP64 fields (8 x 16^3, 8 x 32^3, 4 x 48x64x32, 4 x 64^3), "
+                        f"{K} RK3 steps per graph launch, {reps} launches, {branches} graph branches",
+            "field_steps_per_s_graph": n / t_graph, "field_steps_per_s_eager_1_stream": n / t_eager[1],
+            "field_steps_per_s_eager_branch_streams": n / t_eager[branches],
+            "speedup_vs_eager_1_stream": t_eager[1] / t_graph,
+            "speedup_vs_eager_branch_streams": t_eager[branches] / t_graph,
+            "bitwise_same_as_eager": bool(same)}
+
+
 def run_serving(n_req, rate, seed, precision, max_active, chunk, rm_fraction=0.15):
     """Served mixed-size stream (Poisson arrivals, N in {32..192}, Re ~ U(50,500),
     T ~ U(0.5,5); a quarter J.v requests with T=1 and a share of return-map
@@ -456,6 +513,10 @@ def main():
         extra["fd_jv_batch"] = run_fd_jv_batch(seed=args.seed)
         print(f"[bench] batched FD J.v: {extra['fd_jv_batch']['jv_per_s_batched']:.1f} J.v/s "
               f"({extra['fd_jv_batch']['speedup']:.2f}x one at a time)", file=sys.stderr, flush=True)
+        extra["step_graph"] = run_step_graph(seed=args.seed)
+        print(f"[bench] uneven fields in one graph: {extra['step_graph']['field_steps_per_s_graph']:.0f} field-steps/s "
+              f"({extra['step_graph']['speedup_vs_eager_1_stream']:.2f}x eager on one stream, "
+              f"{extra['step_graph']['speedup_vs_eager_branch_streams']:.2f}x on as many streams as branches)", file=sys.stderr, flush=True)
         if N == 256:
             M = 512
             u512 = pad_spectrum(u0, M)
@@ -520,6 +581,7 @@ def main():
                                 dominant_frac_of_copy=dom["GBps"] / cp)
         out["scalar"] = extra["scalar"]
         out["fd_jv_batch"] = extra["fd_jv_batch"]
+        out["step_graph"] = extra["step_graph"]
         out["tangent_map"] = {"workload": f"{N}^3 RK3_SSP_UV steps (base + one perturbation)",
                               "legs": extra["tangent"]}
         if "n512" in extra:

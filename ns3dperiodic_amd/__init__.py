@@ -3,6 +3,6 @@
 The compute lives in libns3d_hip.so (C ABI: include/ns3d.h); `NS3D` is the
 host-side mirror of the reference operators over that ABI.
 """
-from .solver import NS3D  # noqa: F401
+from .solver import NS3D, StepGraph  # noqa: F401
 
-__all__ = ["NS3D"]
+__all__ = ["NS3D", "StepGraph"]

@@ -4,7 +4,9 @@ The graph is captured once and replayed several times, with unrelated eager
 torch work and host synchronisations between the replays; every replay must
 reproduce the eager kernels' result bit for bit (same kernels, same order).
 A re-capture (changed dt) and the invalidation after a scratch reallocation
-(B(u,v) doubles the context's scratch) are checked the same way.
+(B(u,v) doubles the context's scratch) are checked the same way, and so is
+StepGraph, which captures the steps of several contexts of uneven sizes into
+one graph of concurrent branches.
 """
 import numpy as np
 import pytest
@@ -56,3 +58,39 @@ def test_graph_replays_match_eager(shape, precision):
         assert torch.equal(a, b), "graph after scratch reallocation differs"
     assert all(bool(torch.isfinite(c).all()) for c in ug)
     assert np.isfinite(g.get_kinetic_energy([g.iFFTN(c) for c in ug]))
+
+
+def test_step_graph_of_uneven_sizes():
+    """One captured graph advancing three contexts of different sizes and
+    precisions on concurrent branches (StepGraph, two branches for three
+    items, so one branch runs two contexts in order): four replays with eager
+    work and host syncs between them, each state bitwise equal to stepping
+    it eagerly."""
+    import torch
+    from ns3dperiodic_amd import NS3D, fields
+    from ns3dperiodic_amd import StepGraph
+    print("\n[case] step graph of 32^3 fp64 + 48x64x32 fp32 + 16^3 fp64", flush=True)
+    specs = [((32, 32, 32), "fp64", True), ((48, 64, 32), "fp32", False), ((16, 16, 16), "fp64", True)]
+    items, refs = [], []
+    for i, (sh, prec, forced) in enumerate(specs):
+        g = NS3D(*sh, Re=200.0, dt=2e-3, precision=prec)
+        e = NS3D(*sh, Re=200.0, dt=2e-3, precision=prec)
+        u = [torch.from_numpy(c).cuda() for c in fields.random_solenoidal(*sh, seed=30 + i)]
+        f = [torch.from_numpy(c).cuda() for c in fields.kolmogorov_forcing_sin(*sh, n=2, amp=0.1)] if forced else None
+        items.append((g, u, f))
+        refs.append((e, [c.clone() for c in u], f))
+    K = 3
+    sg = StepGraph(items, steps=K, branches=2)
+    for rep in range(4):
+        sg.replay()
+        torch.cuda.synchronize()
+        junk = torch.randn(1 << 20, device="cuda").sum()      # unrelated eager work between replays
+        for e, r, f in refs:
+            e.RK3_SSP(r, f, steps=K)
+        torch.cuda.synchronize()
+        for (g, u, _), (_, r, _) in zip(items, refs):
+            for a, b in zip(u, r):
+                assert torch.equal(a, b), rep
+        del junk
+    for g, _, _ in items + refs:
+        g.close()
