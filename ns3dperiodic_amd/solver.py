@@ -369,7 +369,10 @@ class StepGraph:
     buffer the step uses exists (capture allows no allocation) and the states
     are left untouched.  The step launches no memset, so the graph holds
     kernels only.  Each item's result is bitwise the one its context gives
-    eagerly.
+    eagerly.  The graph holds the contexts' buffer addresses and dt/Re as they
+    were at capture: ``replay()`` raises (and launches nothing) once a
+    context was closed, reallocated its workspace (a bilinear B(u,v) call or
+    a larger batch grows it) or changed dt or Re; build a new StepGraph then.
     """
 
     def __init__(self, items, steps=1, branches=12, stream=None):
@@ -394,6 +397,17 @@ class StepGraph:
             for s in self.branch_streams:
                 self.stream.wait_stream(s)
         torch.cuda.synchronize()
+        self._keys = [self._key(g) for g, _, _ in self.items]
+
+    @staticmethod
+    def _key(g):
+        if not getattr(g, "ctx", None):
+            return None
+        return (g.ctx.value, int(g.lib.ns3d_workspace_bytes(g.ctx)), g.dt, g.Re)
 
     def replay(self):
+        for i, ((g, _, _), k) in enumerate(zip(self.items, self._keys)):
+            if self._key(g) != k:
+                raise RuntimeError(f"StepGraph: context {i} ({g.Nx}x{g.Ny}x{g.Nz}) was closed, reallocated its "
+                                   "workspace or changed dt/Re since the capture; build a new StepGraph")
         self.graph.replay()

@@ -92,5 +92,13 @@ def test_step_graph_of_uneven_sizes():
             for a, b in zip(u, r):
                 assert torch.equal(a, b), rep
         del junk
+    # a context that changes after the capture makes replay() refuse, not replay stale buffers
+    items[1][0].set_params(dt=1e-3)
+    with pytest.raises(RuntimeError, match="build a new StepGraph"):
+        sg.replay()
+    items[1][0].set_params(dt=2e-3)
+    items[0][0].calculate_convolution_2p3_UV(items[0][1], items[0][1])   # B(u,v) grows context 0's workspace
+    with pytest.raises(RuntimeError, match="build a new StepGraph"):
+        sg.replay()
     for g, _, _ in items + refs:
         g.close()
