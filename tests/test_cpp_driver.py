@@ -9,11 +9,12 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "tests", "cpp", "shapiro_driver.cpp")
+SRC_GRAPH = os.path.join(ROOT, "tests", "cpp", "multi_graph_driver.cpp")
 
 
-def _compile(out):
+def _compile(out, src=SRC):
     cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-           "-I" + os.path.join(ROOT, "include"), SRC, "-o", str(out), "-L" + os.path.join(ROOT, "ns3dperiodic_amd"),
+           "-I" + os.path.join(ROOT, "include"), src, "-o", str(out), "-L" + os.path.join(ROOT, "ns3dperiodic_amd"),
            "-lns3d_hip", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath," + os.path.join(ROOT, "ns3dperiodic_amd")]
     subprocess.run(cmd, check=True)
 
@@ -22,6 +23,7 @@ def test_driver_compiles(tmp_path):
     if not os.path.exists(os.path.join(ROOT, "ns3dperiodic_amd", "libns3d_hip.so")):
         pytest.skip("libns3d_hip.so is not built")
     _compile(tmp_path / "shapiro_driver")
+    _compile(tmp_path / "multi_graph_driver", SRC_GRAPH)
 
 
 @pytest.mark.gpu
@@ -41,3 +43,21 @@ def test_driver_runs_shapiro_32(tmp_path):
     print(f"[cpp] shapiro L-inf: C++ {err_cpp:.17e}, Python {err_py:.17e}", flush=True)
     assert err_cpp < 1e-12
     assert abs(err_cpp - err_py) <= 1e-15
+
+
+@pytest.mark.gpu
+def test_driver_multi_size_graph(tmp_path):
+    """tests/cpp/multi_graph_driver.cpp: a 32^3 and a 16^3 context captured
+    from two forked streams into one HIP graph (plain C ABI), five replays of
+    10 steps bitwise equal to 50 eager steps, and Shapiro-exact."""
+    print("\n[case] C++ driver, 32^3 + 16^3 contexts in one graph", flush=True)
+    exe = tmp_path / "multi_graph_driver"
+    _compile(exe, SRC_GRAPH)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stderr
+    lines = [ln.split() for ln in r.stdout.splitlines()]
+    assert ["bitwise_equal", "1"] in lines
+    errs = {int(p[1]): float(p[2]) for p in lines if p[0] == "shapiro_linf"}
+    assert set(errs) == {16, 32}
+    assert all(e < 1e-12 for e in errs.values()), errs
