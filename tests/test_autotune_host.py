@@ -41,7 +41,7 @@ def test_select_rule(tmp_path, monkeypatch):
     ms[f"NS3D_PXFWD32={alt32}@256|fp32"] = 3.05                      # 0.3% on the fp32+fp32s sum: kept out
     ms[f"NS3D_PXFWD32={alt32}@256|fp32s"] = 2.74
     alt3 = next(c for c in autotune.SPACE["NS3D_R3_PYINV64"][1] if c != base["NS3D_R3_PYINV64"])
-    ms[f"NS3D_R3_PYINV64={alt3}@192|fp64"] = 4.50                    # radix-3 group, judged at 192^3
+    ms[f"NS3D_R3_PYINV64={alt3}@192|fp64"] = 4.50                    # radix-3 group, timed at 192^3
     json.dump({"N": 256, "N3": 192, "dt": {"pow2": 1.9e-3, "r3": 2.5e-3}, "steps": 20, "gpu": "test",
                "ms_per_step": ms}, open(tmp_path / "times.json", "w"))
     autotune.cmd_select(argparse.Namespace(out=str(tmp_path), margin=0.005))
