@@ -303,8 +303,8 @@ def run_step_graph(K=20, reps=10, branches=12, seed=1):
     """Many small fields of uneven sizes advanced by one captured HIP graph
     (StepGraph: K RK3 steps of every field per replay, fields dealt to
     ``branches`` concurrent graph branches) against the same contexts stepped
-    eagerly, K steps each, on one stream and spread over ``branches``
-    streams.  FP64, Kolmogorov forcing f_x = 0.1 sin 2y on every field."""
+    eagerly, K steps each, on one stream and on ``branches`` streams with the
+    graph's assignment of fields to branches.  FP64, Kolmogorov forcing f_x = 0.1 sin 2y on every field."""
     import torch
     from ns3dperiodic_amd import NS3D, StepGraph, fields
     shapes = [(32, 32, 32)] * 8 + [(48, 64, 32)] * 4 + [(64, 64, 64)] * 4 + [(16, 16, 16)] * 8
@@ -330,8 +330,8 @@ def run_step_graph(K=20, reps=10, branches=12, seed=1):
     t_graph = time.perf_counter() - t0
     t_eager = {}
     for ns in (1, branches):
-        for i, (e, _, _) in enumerate(refs):
-            e.set_stream(streams[i % ns])
+        for i, (e, _, _) in enumerate(refs):            # the graph's branch assignment
+            e.set_stream(streams[sg.branch_of[i] if ns > 1 else 0])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
