@@ -381,14 +381,7 @@ class StepGraph:
         self.stream = stream if stream is not None else torch.cuda.Stream()
         nb = max(1, min(int(branches), len(self.items)))
         self.branch_streams = [torch.cuda.Stream() for _ in range(nb)]
-        # longest first onto the least-loaded branch; a step's cost taken as its
-        # grid volume plus a fixed floor (a small grid's kernels cost their latency)
-        load = [0] * nb
-        self.branch_of = [0] * len(self.items)
-        for i in sorted(range(len(self.items)), key=lambda i: -self._cost(self.items[i][0])):
-            b = min(range(nb), key=lambda b: load[b])
-            self.branch_of[i] = b
-            load[b] += self._cost(self.items[i][0])
+        self.branch_of = self.assign([self._cost(g) for g, _, _ in self.items], nb)
         torch.cuda.synchronize()
         for i, (g, u, f) in enumerate(self.items):
             s = self.branch_streams[self.branch_of[i]]
@@ -409,7 +402,20 @@ class StepGraph:
 
     @staticmethod
     def _cost(g):
+        """A step's cost: grid volume plus a fixed floor (a small grid's kernels cost their latency)."""
         return g.Nx * g.Ny * g.Nz + 32 ** 3
+
+    @staticmethod
+    def assign(costs, nb):
+        """Branch of each item: largest cost first, each onto the branch with
+        the least total cost so far (lowest index on ties)."""
+        load = [0] * nb
+        out = [0] * len(costs)
+        for i in sorted(range(len(costs)), key=lambda i: (-costs[i], i)):
+            b = min(range(nb), key=lambda b: (load[b], b))
+            out[i] = b
+            load[b] += costs[i]
+        return out
 
     @staticmethod
     def _key(g):
