@@ -287,16 +287,71 @@ def copy_peak_gbs(reps=5, kinds=("copy", "scale")):
             "kernel": best, "GBps_by_kernel": {k: 2 * n * 8 / (v * 1e-3) / 1e9 for k, v in rates.items()}}
 
 
+def host_cpu():
+    """CPU model, logical CPUs and the CPUs this process may run on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "allowed": allowed}
+
+
 def cpu_baseline(N, u0, Re, dt, steps, threads):
+    """The oracle (FP64 C restatement, OpenMP) on the same workload: one
+    untimed warm-up step, then `steps` steps timed one by one; the median
+    step time is the baseline.  Returns the summary and the oracle's state
+    after the warm-up step and after all steps (for the parity check)."""
     from oracle.oracle import Oracle
     o = Oracle(N, Re=Re, dt=dt, nthreads=threads)
     zero = [np.zeros(o.cshape, np.complex128)] * 3
-    t0 = time.perf_counter()
-    o.rk3(u0, zero, steps=steps)
-    dt_s = time.perf_counter() - t0
-    return {"value": steps / dt_s, "unit": "RK3 steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} RK3 step(s) of the same {N}^3 FP64 workload, oracle C restatement "
-                      f"(OpenMP, own mixed-radix FFT, not FFTW), {dt_s:.1f} s"}
+    u1 = o.rk3(u0, zero, steps=1)                      # warm-up (page-in, thread pool)
+    u = u1
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        u = o.rk3(u, zero, steps=1)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    cpu = host_cpu()
+    out = {"value": 1.0 / med, "unit": "RK3 steps/s", "cores": threads, "kind": "port",
+           "sample": f"median of {steps} RK3 steps (after 1 warm-up step) of the same {N}^3 FP64 workload, "
+                     f"oracle C restatement (OpenMP, own mixed-radix FFT, not FFTW), {sum(times):.1f} s timed",
+           "median_s_per_step": med, "min_s_per_step": float(min(times)), "max_s_per_step": float(max(times)),
+           "threads": threads, "host_cpu_model": cpu["model"], "host_nproc": cpu["nproc"],
+           "host_cpus_allowed": cpu["allowed"]}
+    return out, u1, u
+
+
+def gpu_parity(N, u0, Re, dt, steps, u1_ref, un_ref):
+    """The GPU FP64 path from the same initial bits as the oracle: relative L2
+    distance after 1 step and after 1 + steps steps (the bar is 1e-12 per step)."""
+    import torch
+    from ns3dperiodic_amd import NS3D
+    g = NS3D(N, Re=Re, dt=dt, precision="fp64")
+    u = [torch.from_numpy(c).to("cuda:0") for c in u0]
+    g.RK3_SSP(u, None, steps=1)
+    torch.cuda.synchronize()
+
+    def rel(a, b):
+        num = sum(float(np.sum(np.abs(x.cpu().numpy() - y) ** 2)) for x, y in zip(a, b))
+        return math.sqrt(num / sum(float(np.sum(np.abs(y) ** 2)) for y in b))
+    e1 = rel(u, u1_ref)
+    g.RK3_SSP(u, None, steps=steps)
+    torch.cuda.synchronize()
+    en = rel(u, un_ref)
+    g.close()
+    del u
+    torch.cuda.empty_cache()
+    return {"steps_compared": [1, 1 + steps], "rel_l2_after_1_step": e1, f"rel_l2_after_{1 + steps}_steps": en,
+            "bar_per_step": 1e-12, "pass": bool(e1 <= 1e-12 and en <= 1e-12 * (1 + steps))}
 
 
 def run_step_graph(K=20, reps=10, branches=12, seed=1):
@@ -442,7 +497,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32", "fp32s"], default="fp64")
     ap.add_argument("--secondary", default="fp32,fp32s",
                     help="comma list of further precisions to time beside the primary (fp64, fp32, fp32s) or none")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="oracle steps timed one by one after a warm-up step (the CPU baseline is their median)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--serve-requests", type=int, default=24,
@@ -540,11 +596,15 @@ def main():
                                   "scaled by a plain-copy calibration)" if traffic else None,
                 "alg_bytes_per_launch": dom["alg_bytes"], "avg_launch_ms": dom["avg_ms"]}
 
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu:
-        threads = min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count(), os.cpu_count(), 16)
-        cpu = cpu_baseline(N, u0, Re, dt, args.cpu_steps, threads)
-        print(f"[bench] cpu: {cpu['value']:.4f} steps/s", file=sys.stderr, flush=True)
+        # every core this process may use, unless OMP_NUM_THREADS sets the host's share
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or host_cpu()["allowed"]
+        cpu, u1_ref, un_ref = cpu_baseline(N, u0, Re, dt, args.cpu_steps, threads)
+        print(f"[bench] cpu: {cpu['value']:.4f} steps/s ({threads} threads)", file=sys.stderr, flush=True)
+        parity = gpu_parity(N, u0, Re, dt, args.cpu_steps, u1_ref, un_ref)
+        del u1_ref, un_ref
+        print(f"[bench] parity vs oracle: {parity}", file=sys.stderr, flush=True)
 
     serving = None
     if args.serve_requests > 0:
@@ -568,6 +628,7 @@ def main():
                    "N": N, "Re": Re, "dt": dt, "global_batch": 1, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity_vs_oracle": parity,
         "step_roofline": {"alg_bytes": prim["step_alg_bytes"], "achieved": prim["step_GBps"],
                           "frac": prim["step_GBps"] / HBM_PEAK_GBS},
         "passes": prim["passes"],

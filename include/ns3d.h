@@ -50,6 +50,10 @@ typedef struct { double x, y; } ns3d_cplx;
 #define NS3D_FLAG_MASK_NYQUIST 0x4u /* 2/3 mask relative to each axis's Nyquist wavenumber (alias-free);
                                        the reference normalises by (Nx, Ny, Mz) (math_support.cu:471-519),
                                        which leaves the x and y axes untruncated */
+#define NS3D_FLAG_CHECK_FINITE 0x8u /* the RK3 entry points (ns3d_rk3_ssp, _steps, _batch, _graph, _scalar) end
+                                       by synchronising the stream and reading (and clearing) the non-finite
+                                       flag: NS3D_ERR_NONFINITE if any stage wrote a NaN or Inf.  The reference
+                                       checks for NaN in its time loop (NS3D_periodic.cu:1357-1360) */
 
 typedef struct {
     int Nx, Ny, Nz;        /* physical grid, each in {8,16,32,48,64,96,128,192,256,512} */
@@ -73,6 +77,11 @@ int ns3d_last_error(const ns3d_ctx *ctx);          /* status of the last void-re
 const char *ns3d_status_string(int status);
 int ns3d_supported_size(int n);                    /* 1 if an FFT plan for length n is compiled */
 long long ns3d_workspace_bytes(const ns3d_ctx *ctx);
+/* Count of reallocations of the buffers a captured RK3 step addresses (the fused passes' scratch,
+ * the batch stage buffers): a graph captured from this context's steps stays valid while it is
+ * unchanged.  Growth of other buffers (J.v unpacking, WENO scratch) does not count.  No reference
+ * counterpart. */
+long long ns3d_workspace_generation(const ns3d_ctx *ctx);
 
 /* ---- a4: FFT layer (math_support.cu:301-345) ---- */
 int ns3d_fft_r2c(ns3d_ctx *ctx, const double *in, ns3d_cplx *out);              /* FFTN_Device */
@@ -188,6 +197,18 @@ void ns3d_jv_exponential(void *ctx, void *vec_in, void *vec_out);              /
  *      returned on the host (synchronous, like the reference). ---- */
 int ns3d_reduce_minmax(ns3d_ctx *ctx, long long n, const double *in, double *out2_host);  /* compute_reduction: [min,max] */
 int ns3d_reduce_sum(ns3d_ctx *ctx, long long n, const double *in, double *out_host);      /* reduction_sum */
+/* NS3D_ERR_NONFINITE if any of the n doubles is NaN or Inf, else NS3D_OK (a spectral field of
+ * nc modes is 2*nc doubles).  check_nans_kernel (Source/cuda_supp.cu:242-325), which returns -1 /
+ * exits on NaN; Inf is reported too. */
+int ns3d_check_finite(ns3d_ctx *ctx, long long n, const double *x);
+/* The fused stage epilogues (every ns3d_stage / RK3 / J.v / update call through the fused passes)
+ * and the passive-scalar stage set a sticky per-context flag when they write a NaN or Inf.
+ * ns3d_finite_status synchronises the stream, returns NS3D_ERR_NONFINITE if the flag is set
+ * (else NS3D_OK) and clears it if `clear`; ns3d_finite_flag_async enqueues a copy of the flag
+ * (0 or 1) to host memory on the context's stream (no synchronisation; pinned memory keeps it
+ * asynchronous).  No reference counterpart beyond the NaN checks above. */
+int ns3d_finite_status(ns3d_ctx *ctx, int clear);
+int ns3d_finite_flag_async(ns3d_ctx *ctx, int *host_flag);
 int ns3d_kinetic_energy(ns3d_ctx *ctx, const double *ux, const double *uy, const double *uz, double *out_host);
 int ns3d_dissipation(ns3d_ctx *ctx, const double *ux, const double *uy, const double *uz, double *out_host);
 int ns3d_cfl_dt(ns3d_ctx *ctx, double cfl, const double *ux, const double *uy, const double *uz, double *dt_host);

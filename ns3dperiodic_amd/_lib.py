@@ -16,6 +16,8 @@ NS3D_FP32_STATE = 2
 NS3D_FLAG_NO_IPQ = 0x1
 NS3D_FLAG_WENO_ZFIX = 0x2
 NS3D_FLAG_MASK_NYQUIST = 0x4
+NS3D_FLAG_CHECK_FINITE = 0x8
+NS3D_ERR_NONFINITE = -5
 UPD_STAGE, UPD_UV_RHS, UPD_UV, UPD_IUV, UPD_RHS = 0, 1, 2, 3, 4
 
 
@@ -44,6 +46,7 @@ SIGNATURES = {
     "ns3d_status_string": (ctypes.c_char_p, [I]),
     "ns3d_supported_size": (I, [I]),
     "ns3d_workspace_bytes": (LL, [P]),
+    "ns3d_workspace_generation": (LL, [P]),
     "ns3d_fft_r2c": (I, [P, P, P]),
     "ns3d_fft_c2r": (I, [P, P, P]),
     "ns3d_fft_c2r_unscaled": (I, [P, P, P]),
@@ -80,6 +83,9 @@ SIGNATURES = {
     "ns3d_jv_exponential": (None, [P, P, P]),
     "ns3d_reduce_minmax": (I, [P, LL, P, DP]),
     "ns3d_reduce_sum": (I, [P, LL, P, DP]),
+    "ns3d_check_finite": (I, [P, LL, P]),
+    "ns3d_finite_status": (I, [P, I]),
+    "ns3d_finite_flag_async": (I, [P, P]),
     "ns3d_vec_dot": (I, [P, LL, P, P, DP]),
     "ns3d_vec_gemv_t": (I, [P, LL, I, P, LL, P, DP]),
     "ns3d_vec_gemv": (I, [P, LL, I, P, LL, DP, D, D, P]),
@@ -104,6 +110,10 @@ class NS3DError(RuntimeError):
     pass
 
 
+class NonFiniteError(NS3DError):
+    """NS3D_ERR_NONFINITE: a checking call found a NaN or Inf."""
+
+
 def load():
     """Load libns3d_hip.so and declare every entry point; raises if it is missing."""
     global _lib
@@ -123,4 +133,5 @@ def load():
 def check(status, what=""):
     if status != NS3D_OK:
         msg = load().ns3d_status_string(status).decode()
-        raise NS3DError(f"{what}: status {status} ({msg})")
+        cls = NonFiniteError if status == NS3D_ERR_NONFINITE else NS3DError
+        raise cls(f"{what}: status {status} ({msg})")

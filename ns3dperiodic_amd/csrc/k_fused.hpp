@@ -685,7 +685,17 @@ struct EpiArgs {
     int ny, mw, mu0, mo, mf;
     // batch member strides (elements) of w, u0 and out; f is shared by the members
     long bw, bu0, bo;
+    // sticky non-finite flag: set to 1 by any lane that writes a NaN or Inf (may be null)
+    int* nf;
 };
+
+// Non-finite flag of the fused epilogues and the scalar stage: every lane that
+// produced a NaN or Inf stores 1 (a plain vector store; all writers store the
+// same value).  The flag is read and cleared by the host.
+__device__ __forceinline__ void flag_nonfinite(int* nf, bool bad) {
+    if (bad && nf) *(volatile int*)nf = 1;
+}
+template <typename T> __device__ __forceinline__ bool finite2(cplx<T> a) { return isfinite(a.x) && isfinite(a.y); }
 
 // Projection/diffusion/update at one mode (Source/diffusion.cu:96-365).
 template <int MODE>
@@ -697,8 +707,10 @@ __device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& ta
         ea.out[0][go] = Qx;
         ea.out[1][go] = Qy;
         ea.out[2][go] = Qz;
+        flag_nonfinite(ea.nf, !(finite2(Qx) && finite2(Qy) && finite2(Qz)));
         return;
     }
+    bool bad = false;
     const ProjCoef c = proj_coef(tab, j, k, l);
     const double AQr[3] = {c.a11 * Qx.x + c.a12 * Qy.x + c.a13 * Qz.x, c.a12 * Qx.x + c.a22 * Qy.x + c.a23 * Qz.x,
                            c.a13 * Qx.x + c.a23 * Qy.x + c.a33 * Qz.x};
@@ -721,7 +733,9 @@ __device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& ta
                 r.x = ea.a0 * u0.x + ea.a1 * r.x;
                 r.y = ea.a0 * u0.y + ea.a1 * r.y;
             }
-            reinterpret_cast<cplx<float>*>(ea.out[b])[go] = cvt<float>(r);
+            const cplx<float> rf = cvt<float>(r);
+            reinterpret_cast<cplx<float>*>(ea.out[b])[go] = rf;
+            bad |= !finite2(rf);
             continue;
         }
         const cplx<double> w = ea.w[b][gw];
@@ -737,6 +751,7 @@ __device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& ta
                 r.y = ea.a0 * u0.y + ea.a1 * r.y;
             }
             ea.out[b][go] = r;
+            bad |= !finite2(r);
         } else if (MODE == EPI_RHS) {
             const double G = (-c.din / Re);
             cplx<double> f = mk<double>(0.0, 0.0);
@@ -744,23 +759,28 @@ __device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& ta
             r.x = AQr[b] + G * w.x + f.x - Q[b].x;
             r.y = AQi[b] + G * w.y + f.y - Q[b].y;
             ea.out[b][go] = r;
+            bad |= !finite2(r);
         } else if (MODE == EPI_UV) {
             const double G = (1.0 - c.din * dt / Re);
             r.x = dt * AQr[b] + G * w.x - dt * Q[b].x;
             r.y = dt * AQi[b] + G * w.y - dt * Q[b].y;
             ea.w[b][gw] = r;
+            bad |= !finite2(r);
         } else if (MODE == EPI_UV_RHS) {
             const double G = (-c.din / Re);
             r.x = AQr[b] + G * w.x - Q[b].x;
             r.y = AQi[b] + G * w.y - Q[b].y;
             ea.w[b][gw] = r;
+            bad |= !finite2(r);
         } else {  // EPI_IUV
             const double G = (1.0 + c.din * dt / Re);
             r.x = (dt * AQr[b] + w.x - dt * Q[b].x) / G;
             r.y = (dt * AQi[b] + w.y - dt * Q[b].y) / G;
             ea.w[b][gw] = r;
+            bad |= !finite2(r);
         }
     }
+    flag_nonfinite(ea.nf, bad);
 }
 
 // set_to_zero_real (Source/advection_2_3.cu:35-70)

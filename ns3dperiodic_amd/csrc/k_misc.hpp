@@ -311,7 +311,7 @@ template <int SCHEME, typename TU>
 __global__ void __launch_bounds__(256)
 k_weno_stage(const TU* __restrict__ u, long nr, const double* __restrict__ x, const double* __restrict__ xn,
              double* __restrict__ out, Grid3 G, double rdx, double rdy, double rdz, int zfix, double dt,
-             double a0, double a1) {
+             double a0, double a1, int* nf) {
     const long id = weno_index();
     if (id >= nr) return;
     const int l = (int)(id % G.Nz);
@@ -319,7 +319,13 @@ k_weno_stage(const TU* __restrict__ u, long nr, const double* __restrict__ x, co
     const int k = (int)(t1 % G.Ny), j = (int)(t1 / G.Ny);
     const double adv = weno_point<SCHEME>(u, u + nr, u + 2 * nr, x, G, id, j, k, l, rdx, rdy, rdz, zfix);
     const double e = x[id] - dt * adv;
-    out[id] = xn ? a0 * xn[id] + a1 * e : a1 * e;
+    const double o = xn ? a0 * xn[id] + a1 * e : a1 * e;
+    out[id] = o;
+    flag_nonfinite(nf, !isfinite(o));
+}
+
+__global__ void k_set_int(int* __restrict__ p, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
 }
 
 // ------------------------------------------------------- a11: reductions
@@ -369,9 +375,11 @@ __device__ __forceinline__ void block_finish(double a, double b, double* __restr
 // SRC_SPEC_*: over the nc half-spectrum modes of three spectral fields (a, b,
 // c read as interleaved complex), each mode weighted by its Hermitian
 // multiplicity (1 on the l = 0 and l = Nz/2 planes, else 2).
+// SRC_NONFINITE: 1 for each NaN or Inf among n doubles, else 0 (check_nans_kernel,
+// Source/cuda_supp.cu:242-325, which tests isnan only; Inf is counted here too).
 enum RedSrc {
     SRC_ARRAY = 0, SRC_ENERGY = 1, SRC_DISSIPATION = 2, SRC_CFL = 3, SRC_SHAPIRO = 4,
-    SRC_SPEC_ENERGY = 5, SRC_SPEC_DISSIPATION = 6
+    SRC_SPEC_ENERGY = 5, SRC_SPEC_DISSIPATION = 6, SRC_NONFINITE = 7
 };
 struct RedArgs {
     const double* a;
@@ -388,6 +396,7 @@ struct RedArgs {
 template <int SRC>
 __device__ __forceinline__ double red_value(const RedArgs& r, long id) {
     if (SRC == SRC_ARRAY) return r.a[id];
+    if (SRC == SRC_NONFINITE) return isfinite(r.a[id]) ? 0.0 : 1.0;
     if (SRC == SRC_SPEC_ENERGY || SRC == SRC_SPEC_DISSIPATION) {
         const int l = (int)(id % r.Mz);
         const long t1 = id / r.Mz;

@@ -173,6 +173,8 @@ struct ns3d_ctx {
     // bulk scratch (A|B regions) in bulk precision; capacity in fields
     void* scratch = nullptr;
     int scratch_fields = 0;
+    // incremented whenever a buffer that a captured RK3 step addresses is reallocated
+    long long generation = 0;
     // FP64 stage buffers and a Q^ buffer
     zd* stage = nullptr;  // 9 fields: u1[3], u2[3], q[3]
     zd* extra = nullptr;  // 6 fields: tangent map v1[3] + spare
@@ -214,6 +216,9 @@ struct ns3d_ctx {
     std::vector<std::pair<int, int>> ev_used;  // (pass id, index of start event; stop = +1)
     hipEvent_t ev_open = nullptr;
     int ev_open_pass = -1;
+    // sticky non-finite flag written by the fused epilogues and the scalar stage
+    int* nf = nullptr;
+    int* nf_host = nullptr;  // pinned
 };
 
 namespace {
@@ -381,6 +386,7 @@ void drop_graph(ns3d_ctx* c) {
 int ensure_scratch(ns3d_ctx* c, int fields) {
     if (c->scratch_fields >= fields) return NS3D_OK;
     drop_graph(c);
+    if (c->scratch) c->generation++;
     if (c->scratch) {
         hipStreamSynchronize(c->stream);
         hipFree(c->scratch);
@@ -539,6 +545,7 @@ EpiArgs make_epi(const ns3d_ctx* c) {
     ea.ipq = (c->flags & NS3D_FLAG_NO_IPQ) ? 0 : 1;
     ea.ny = c->Ny;
     ea.mw = ea.mu0 = ea.mo = ea.mf = c->Mz;
+    ea.nf = c->nf;
     return ea;
 }
 
@@ -656,6 +663,7 @@ int rk3_one_batch(ns3d_ctx* c, int nb, zd* u, const zd* const f[3]) {
     if (c->bstage_members < nb) {
         drop_graph(c);
         if (c->bstage) {
+            c->generation++;
             hipStreamSynchronize(c->stream);
             hipFree(c->bstage);
             c->bstage = nullptr;
@@ -861,7 +869,7 @@ void launch_weno_stage(ns3d_ctx* c, const double* xs, const double* xn, double* 
     const int zfix = (c->flags & NS3D_FLAG_WENO_ZFIX) ? 1 : 0;
     hipLaunchKernelGGL((k_weno_stage<SCHEME, TU>), xgrid(nblk(c->nr, 256)), dim3(256), 0, c->stream,
                        (const TU*)c->uphys, c->nr, xs, xn, out, G, c->Nx / c->Lx, c->Ny / c->Ly, c->Nz / c->Lz, zfix,
-                       c->dt, a0, a1);
+                       c->dt, a0, a1, c->nf);
 }
 template <typename TU>
 void launch_weno_stage_s(ns3d_ctx* c, int scheme, const double* xs, const double* xn, double* out, double a0,
@@ -898,6 +906,25 @@ int rk3_scalar_one(ns3d_ctx* c, zd* const u[3], const zd* const f[3], double* th
         st = weno_stage(c, scheme, th2, th, th, 1.0 / 3.0, 2.0 / 3.0);
     c->uphys_out = nullptr;
     return st;
+}
+
+// Read the sticky non-finite flag (synchronises the context stream); clear it if asked.
+int finite_status(ns3d_ctx* c, bool clear) {
+    HIPCHK(hipMemcpyAsync(c->nf_host, c->nf, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int v = *c->nf_host;
+    if (v && clear) {
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->nf, 0);
+        HIPCHK(hipGetLastError());
+    }
+    return v ? NS3D_ERR_NONFINITE : NS3D_OK;
+}
+
+// Status of an RK3 entry point: with NS3D_FLAG_CHECK_FINITE, a successful call
+// ends with a read (and clear) of the non-finite flag.
+int checked(ns3d_ctx* c, int st) {
+    if (st || !(c->flags & NS3D_FLAG_CHECK_FINITE)) return st;
+    return finite_status(c, true);
 }
 
 int ensure_kv(ns3d_ctx* c) {
@@ -967,6 +994,12 @@ int ns3d_create(const ns3d_config* cfg, ns3d_ctx** out) {
     if (!st) st = build_twiddles(c);
     if (!st && hipMalloc((void**)&c->red, sizeof(double) * (2 * RED_BLOCKS + 2)) != hipSuccess) st = NS3D_ERR_ALLOC;
     if (!st && hipHostMalloc((void**)&c->red_host, 2 * sizeof(double)) != hipSuccess) st = NS3D_ERR_ALLOC;
+    if (!st && hipMalloc((void**)&c->nf, 64 * sizeof(int)) != hipSuccess) st = NS3D_ERR_ALLOC;
+    if (!st && hipHostMalloc((void**)&c->nf_host, sizeof(int)) != hipSuccess) st = NS3D_ERR_ALLOC;
+    if (!st) {
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->nf, 0);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) st = NS3D_ERR_HIP;
+    }
     if (st) {
         ns3d_destroy(c);
         return st;
@@ -999,6 +1032,8 @@ int ns3d_destroy(ns3d_ctx* c) {
     hipFree(c->uphys);
     hipFree(c->theta_stage);
     if (c->red_host) hipHostFree(c->red_host);
+    hipFree(c->nf);
+    if (c->nf_host) hipHostFree(c->nf_host);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
@@ -1026,6 +1061,8 @@ int ns3d_synchronize(ns3d_ctx* c) {
 }
 
 int ns3d_last_error(const ns3d_ctx* c) { return c ? c->last_error : NS3D_ERR_ARG; }
+
+long long ns3d_workspace_generation(const ns3d_ctx* c) { return c ? c->generation : -1; }
 
 long long ns3d_workspace_bytes(const ns3d_ctx* c) {
     if (!c) return -1;
@@ -1212,7 +1249,7 @@ int ns3d_rk3_ssp_steps(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns
     if (!c || !ux || !uy || !uz || nsteps < 0) return NS3D_ERR_ARG;
     zd* u[3] = {(zd*)ux, (zd*)uy, (zd*)uz};
     const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
-    return rk3_steps(c, nsteps, u, f);
+    return checked(c, rk3_steps(c, nsteps, u, f));
 }
 
 int ns3d_rk3_ssp_batch(ns3d_ctx* c, int nbatch, int nsteps, ns3d_cplx* u, const ns3d_cplx* fx, const ns3d_cplx* fy,
@@ -1228,13 +1265,13 @@ int ns3d_rk3_ssp_batch(ns3d_ctx* c, int nbatch, int nsteps, ns3d_cplx* u, const 
             int st = rk3_steps(c, nsteps, m, f);
             if (st) return st;
         }
-        return NS3D_OK;
+        return checked(c, NS3D_OK);
     }
     for (int i = 0; i < nsteps; i++) {
         int st = rk3_one_batch(c, nbatch, ub, f);
         if (st) return st;
     }
-    return NS3D_OK;
+    return checked(c, NS3D_OK);
 }
 
 int ns3d_rk3_ssp_scalar(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns3d_cplx* uz, const ns3d_cplx* fx,
@@ -1246,7 +1283,7 @@ int ns3d_rk3_ssp_scalar(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, n
         int st = rk3_scalar_one(c, u, f, theta, scheme);
         if (st) return st;
     }
-    return NS3D_OK;
+    return checked(c, NS3D_OK);
 }
 
 int ns3d_rk3_ssp_graph(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns3d_cplx* uz, const ns3d_cplx* fx,
@@ -1289,7 +1326,7 @@ int ns3d_rk3_ssp_graph(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns
         c->graph_Re = c->Re;
     }
     HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
-    return NS3D_OK;
+    return checked(c, NS3D_OK);
 }
 
 // ---- a8
@@ -1522,6 +1559,23 @@ int ns3d_reduce_sum(ns3d_ctx* c, long long n, const double* in, double* out) {
     if (!c || !in || !out || n < 0) return NS3D_ERR_ARG;
     RedArgs r = red_args(c, in, nullptr, nullptr);
     return reduce<0, SRC_ARRAY>(c, r, (long)n, out);
+}
+int ns3d_check_finite(ns3d_ctx* c, long long n, const double* x) {
+    if (!c || !x || n < 0) return NS3D_ERR_ARG;
+    RedArgs r = red_args(c, x, nullptr, nullptr);
+    double bad = 0.0;
+    const int st = reduce<0, SRC_NONFINITE>(c, r, (long)n, &bad);
+    if (st) return st;
+    return bad > 0.0 ? NS3D_ERR_NONFINITE : NS3D_OK;
+}
+int ns3d_finite_status(ns3d_ctx* c, int clear) {
+    if (!c) return NS3D_ERR_ARG;
+    return finite_status(c, clear != 0);
+}
+int ns3d_finite_flag_async(ns3d_ctx* c, int* host_flag) {
+    if (!c || !host_flag) return NS3D_ERR_ARG;
+    HIPCHK(hipMemcpyAsync(host_flag, c->nf, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    return NS3D_OK;
 }
 int ns3d_kinetic_energy(ns3d_ctx* c, const double* ux, const double* uy, const double* uz, double* out) {
     if (!c || !ux || !uy || !uz || !out) return NS3D_ERR_ARG;

@@ -118,26 +118,37 @@ def request_inputs(req, energy=0.5):
     return u, fx, v
 
 
+def _rows(N, a):
+    """Modes of one field whose rows of Mz = N/2+1 are padded to a multiple of a."""
+    return N * N * ((N // 2 + 1 + a - 1) // a * a)
+
+
 def field_bytes(N, precision):
-    """HBM held by an integrate request on an N^3 grid: 3 state fields and the
-    forcing x-component (FP64), 9 FP64 RK stage fields and 15 scratch fields
-    in the bulk precision."""
+    """HBM held by an integrate request on an N^3 grid, in the library's layout:
+    3 state fields and the forcing x-component (FP64, reference layout); 9 FP64
+    RK stage fields (rows padded to 8 modes); 15 scratch fields in the bulk
+    precision (rows padded to 8 modes FP64, 16 FP32); for "fp32s" the FP32
+    state and stages of the RK3 loop (9 fields, rows padded to 16) replace the
+    FP64 stage fields."""
     nc = N * N * (N // 2 + 1)
-    bulk = 16 if precision == "fp64" else 8
-    return nc * (16 * (3 + 1 + 9) + bulk * 15)
+    bulk, pad = (16, 8) if precision == "fp64" else (8, 16)
+    b = 16 * 4 * nc + bulk * 15 * _rows(N, pad)
+    # the FP32-state loop keeps its stages in FP32 (no FP64 stage buffer)
+    b += 8 * 9 * _rows(N, 16) if precision == "fp32s" else 16 * 9 * _rows(N, 8)
+    return b
 
 
 def request_bytes(req, precision):
     """field_bytes plus, for J.v, the perturbation (3 FP64 fields), the
-    tangent map's 6 FP64 stage fields and 15 more scratch fields (B(u,v));
-    for a return map, the saved state (3 FP64 fields)."""
+    tangent map's 6 FP64 stage fields (padded rows) and 15 more scratch
+    fields (B(u,v)); for a return map, the saved state (3 FP64 fields)."""
     nc = req.N * req.N * (req.N // 2 + 1)
     b = field_bytes(req.N, precision)
     if req.kind == "return_map":
         b += nc * 16 * 3              # the state saved before each step (bisection restart)
     if req.kind == "jv":
-        bulk = 16 if precision == "fp64" else 8
-        b += nc * (16 * (3 + 6) + bulk * 15)
+        bulk, pad = (16, 8) if precision == "fp64" else (8, 16)
+        b += 16 * 3 * nc + 16 * 6 * _rows(req.N, 8) + bulk * 15 * _rows(req.N, pad)
     return b
 
 

@@ -34,7 +34,7 @@ class NS3D:
 
     def __init__(self, Nx, Ny=None, Nz=None, Lx=2 * math.pi, Ly=2 * math.pi, Lz=2 * math.pi,
                  Re=100.0, dt=1e-3, precision="fp64", ipq=True, weno_zfix=False, device=0, stream=None,
-                 mask_nyquist=False):
+                 mask_nyquist=False, check_finite=False):
         self.lib = _lib.load()
         Ny = Nx if Ny is None else Ny
         Nz = Nx if Nz is None else Nz
@@ -43,7 +43,8 @@ class NS3D:
         self.Re, self.dt = Re, dt
         self.device = torch.device("cuda", device)
         flags = ((0 if ipq else _lib.NS3D_FLAG_NO_IPQ) | (_lib.NS3D_FLAG_WENO_ZFIX if weno_zfix else 0)
-                 | (_lib.NS3D_FLAG_MASK_NYQUIST if mask_nyquist else 0))
+                 | (_lib.NS3D_FLAG_MASK_NYQUIST if mask_nyquist else 0)
+                 | (_lib.NS3D_FLAG_CHECK_FINITE if check_finite else 0))
         # "fp32": FP32 transforms, FP64 state; "fp32s": also FP32 state inside the RK3 loops
         prec = {"fp64": _lib.NS3D_FP64, "fp32": _lib.NS3D_FP32_BULK, "fp32s": _lib.NS3D_FP32_STATE}[precision]
         self.precision = precision
@@ -315,6 +316,30 @@ class NS3D:
         check(self.lib.ns3d_reduce_sum(self.ctx, x.numel(), _ptr(x), ctypes.byref(out)), "ns3d_reduce_sum")
         return out.value
 
+    def check_nans(self, x):
+        """check_nans_kernel: True if every value of x (any float64/complex128 device tensor) is finite."""
+        x = x.contiguous()
+        n = x.numel() * (2 if x.is_complex() else 1)
+        st = self.lib.ns3d_check_finite(self.ctx, n, _ptr(x))
+        if st == _lib.NS3D_ERR_NONFINITE:
+            return False
+        check(st, "ns3d_check_finite")
+        return True
+
+    def finite_status(self, clear=True):
+        """False if a fused stage wrote a NaN or Inf since the flag was last cleared (synchronises)."""
+        st = self.lib.ns3d_finite_status(self.ctx, int(clear))
+        if st == _lib.NS3D_ERR_NONFINITE:
+            return False
+        check(st, "ns3d_finite_status")
+        return True
+
+    def finite_flag_async(self, host_int32):
+        """Enqueue a copy of the non-finite flag (0/1) into a pinned int32 host tensor."""
+        assert host_int32.dtype == torch.int32 and not host_int32.is_cuda
+        check(self.lib.ns3d_finite_flag_async(self.ctx, ctypes.c_void_p(host_int32.data_ptr())),
+              "ns3d_finite_flag_async")
+
     def kinetic_energy_spectral(self, u_hat):
         """0.5 * integral of |u|^2 over the box, exact from the spectral fields."""
         out = ctypes.c_double()
@@ -371,8 +396,15 @@ class StepGraph:
     kernels only.  Each item's result is bitwise the one its context gives
     eagerly.  The graph holds the contexts' buffer addresses and dt/Re as they
     were at capture: ``replay()`` raises (and launches nothing) once a
-    context was closed, reallocated its workspace (a bilinear B(u,v) call or
-    a larger batch grows it) or changed dt or Re; build a new StepGraph then.
+    context was closed, reallocated a buffer the step uses
+    (ns3d_workspace_generation changed: a bilinear B(u,v) call or a larger
+    batch grows the scratch) or changed dt or Re; build a new StepGraph then.
+    Buffers the step does not use (J.v unpacking, WENO scratch) may grow.
+
+    Ordering: ``replay()`` runs the graph on its own stream after all work
+    already queued on the caller's current stream and on every branch stream
+    (where the contexts' eager calls run), and makes both wait for the replay,
+    so eager calls before and after it need no host synchronisation.
     """
 
     def __init__(self, items, steps=1, branches=12, stream=None):
@@ -421,11 +453,19 @@ class StepGraph:
     def _key(g):
         if not getattr(g, "ctx", None):
             return None
-        return (g.ctx.value, int(g.lib.ns3d_workspace_bytes(g.ctx)), g.dt, g.Re)
+        return (g.ctx.value, int(g.lib.ns3d_workspace_generation(g.ctx)), g.dt, g.Re)
 
     def replay(self):
         for i, ((g, _, _), k) in enumerate(zip(self.items, self._keys)):
             if self._key(g) != k:
-                raise RuntimeError(f"StepGraph: context {i} ({g.Nx}x{g.Ny}x{g.Nz}) was closed, reallocated its "
-                                   "workspace or changed dt/Re since the capture; build a new StepGraph")
-        self.graph.replay()
+                raise RuntimeError(f"StepGraph: context {i} ({g.Nx}x{g.Ny}x{g.Nz}) was closed, reallocated a "
+                                   "buffer of its step or changed dt/Re since the capture; build a new StepGraph")
+        cur = torch.cuda.current_stream()
+        self.stream.wait_stream(cur)
+        for s in self.branch_streams:
+            self.stream.wait_stream(s)
+        with torch.cuda.stream(self.stream):
+            self.graph.replay()
+        cur.wait_stream(self.stream)
+        for s in self.branch_streams:
+            s.wait_stream(self.stream)

@@ -22,9 +22,21 @@
  *                           Kolmogorov fixed point.
  *   J.v / tangent map       central finite differences of the nonlinear map.
  *   divergence              closed form on trigonometric fields.
- *   WENO                    polynomial reproduction in the linear-weight
- *                           limit, convergence order on smooth data, a
- *                           brute-force scalar restatement on tiny grids.
+ *   WENO                    convergence order on smooth data; one Euler step
+ *                           of a step function stays within its bounds along
+ *                           each axis (fixes the upwind side); the two cells
+ *                           of a face use the same face value, so
+ *                           sum_j out_j / u_j telescopes to zero along a line
+ *                           (fixes the face-averaged velocity and the face
+ *                           indices).
+ *   passive scalar RK3      theta advected by the exact decaying shear
+ *                           (e^{-t/Re} sin y, 0, 0) against the closed form
+ *                           theta0(x - sin y Re (1 - e^{-t/Re})), third order
+ *                           in time (fixes each stage's velocity and weights).
+ *
+ * or_set_mutation() plants one known wiring mistake (test-only fault injection
+ * on the CPU): the tests use it to show that each pinning check above fails
+ * on the mistake it is meant to catch.  0 = the restatement as written.
  *   reductions              exact min/max; long double sums.
  *   energy/dissipation/CFL  closed forms on trigonometric fields.
  */
@@ -44,6 +56,12 @@ typedef double complex cx;
 #define OR_PI 3.141592653589793238462643383279502884
 
 #define IDX3(j, k, l, N2, N3) (((long)(j) * (N2) + (k)) * (long)(N3) + (l))
+
+/* Planted mistakes (test-only): 1/2/3 downwind faces along x/y/z, 4 cell
+ * velocity in place of the face average, 5 every scalar stage advected by
+ * u^n's velocity, 6 the second scalar stage's SSP weights swapped. */
+static int or_mut = 0;
+void or_set_mutation(int m) { or_mut = m; }
 
 static void set_threads(const or_params *p)
 {
@@ -134,28 +152,53 @@ void or_fft_c2c_1d(int n, int sign, const double *in, double *out)
     plan_free(&pl);
 }
 
-/* C2C along one axis of a complex [A][B][C] array (in place). axis 0, 1 or 2. */
+/* C2C along one axis of a complex [A][B][C] array (in place). axis 0, 1 or 2.
+ * Strided axes are gathered OR_W lines at a time (lines adjacent in the
+ * fastest index), so every cache line read or written is used whole. */
+#define OR_W 8
 static void fft_axis(cx *a, int A, int B, int C, int axis, int sign)
 {
     const int n = axis == 0 ? A : axis == 1 ? B : C;
     plan1d pl;
     plan_init(&pl, n, sign);
-    const long nlines = (long)A * B * C / n;
-    #pragma omp parallel
-    {
-        cx *buf = (cx *)malloc(sizeof(cx) * (size_t)n * 2);
-        cx *res = buf + n;
-        #pragma omp for schedule(static)
-        for (long ln = 0; ln < nlines; ln++) {
-            long base, stride;
-            if (axis == 0) { base = ln; stride = (long)B * C; }
-            else if (axis == 1) { long i = ln / C, c = ln % C; base = i * (long)B * C + c; stride = C; }
-            else { base = ln * C; stride = 1; }
-            for (int q = 0; q < n; q++) buf[q] = a[base + q * stride];
-            fft_line(&pl, buf, res);
-            for (int q = 0; q < n; q++) a[base + q * stride] = res[q];
+    if (axis == 2) {
+        const long nlines = (long)A * B;
+        #pragma omp parallel
+        {
+            cx *res = (cx *)malloc(sizeof(cx) * (size_t)n);
+            #pragma omp for schedule(static)
+            for (long ln = 0; ln < nlines; ln++) {
+                fft_line(&pl, a + ln * C, res);
+                memcpy(a + ln * C, res, sizeof(cx) * (size_t)n);
+            }
+            free(res);
         }
-        free(buf);
+    } else {
+        /* lines (o, c): o = b (axis 0) or i (axis 1); c = fastest index, grouped by OR_W */
+        const long nouter = axis == 0 ? B : A;
+        const long stride = axis == 0 ? (long)B * C : C;
+        const long ngrp = (C + OR_W - 1) / OR_W;
+        #pragma omp parallel
+        {
+            cx *buf = (cx *)malloc(sizeof(cx) * (size_t)n * OR_W);
+            cx *res = (cx *)malloc(sizeof(cx) * (size_t)n);
+            #pragma omp for schedule(static)
+            for (long g = 0; g < nouter * ngrp; g++) {
+                const long o = g / ngrp, c0 = (g % ngrp) * OR_W;
+                const int w = (int)(C - c0 < OR_W ? C - c0 : OR_W);
+                const long base = (axis == 0 ? o * C : o * (long)B * C) + c0;
+                for (int q = 0; q < n; q++)
+                    for (int e = 0; e < w; e++) buf[(long)e * n + q] = a[base + q * stride + e];
+                for (int e = 0; e < w; e++) {
+                    fft_line(&pl, buf + (long)e * n, res);
+                    memcpy(buf + (long)e * n, res, sizeof(cx) * (size_t)n);
+                }
+                for (int q = 0; q < n; q++)
+                    for (int e = 0; e < w; e++) a[base + q * stride + e] = buf[(long)e * n + q];
+            }
+            free(buf);
+            free(res);
+        }
     }
     plan_free(&pl);
 }
@@ -667,19 +710,23 @@ void or_weno_advect(const or_params *p, int scheme, const double *ux, const doub
     for (int j = 0; j < Nx; j++)
         for (int k = 0; k < Ny; k++)
             for (int l = 0; l < Nz; l++) {
-                const double Uw = 0.5 * (at(ux, Nx, Ny, Nz, j, k, l, 0, -1) + at(ux, Nx, Ny, Nz, j, k, l, 0, 0));
-                const double Ue = 0.5 * (at(ux, Nx, Ny, Nz, j, k, l, 0, 0) + at(ux, Nx, Ny, Nz, j, k, l, 0, 1));
+                const long id = IDX3(j, k, l, Ny, Nz);
+                const int cellv = or_mut == 4;
+                double Uw = 0.5 * (at(ux, Nx, Ny, Nz, j, k, l, 0, -1) + at(ux, Nx, Ny, Nz, j, k, l, 0, 0));
+                double Ue = 0.5 * (at(ux, Nx, Ny, Nz, j, k, l, 0, 0) + at(ux, Nx, Ny, Nz, j, k, l, 0, 1));
+                double Un = 0.5 * (at(uy, Nx, Ny, Nz, j, k, l, 1, -1) + at(uy, Nx, Ny, Nz, j, k, l, 1, 0));
+                double Us = 0.5 * (at(uy, Nx, Ny, Nz, j, k, l, 1, 0) + at(uy, Nx, Ny, Nz, j, k, l, 1, 1));
+                double Ur = 0.5 * (at(uz, Nx, Ny, Nz, j, k, l, 2, -1) + at(uz, Nx, Ny, Nz, j, k, l, 2, 0));
+                double Uf = 0.5 * (at(uz, Nx, Ny, Nz, j, k, l, 2, 0) + at(uz, Nx, Ny, Nz, j, k, l, 2, 1));
+                if (cellv) { Uw = Ue = ux[id]; Un = Us = uy[id]; Ur = Uf = uz[id]; }
+                const double gx = or_mut == 1 ? -1.0 : 1.0, gy = or_mut == 2 ? -1.0 : 1.0, gz = or_mut == 3 ? -1.0 : 1.0;
+                Uw *= gx; Ue *= gx; Un *= gy; Us *= gy; Ur *= gz; Uf *= gz;
                 const double sw = Uw >= 0.0 ? face(p, x, wrap(j - 1, Nx), k, l, 0, 0, scheme) : face(p, x, j, k, l, 0, 1, scheme);
                 const double se = Ue >= 0.0 ? face(p, x, j, k, l, 0, 0, scheme) : face(p, x, wrap(j + 1, Nx), k, l, 0, 1, scheme);
-                const double Un = 0.5 * (at(uy, Nx, Ny, Nz, j, k, l, 1, -1) + at(uy, Nx, Ny, Nz, j, k, l, 1, 0));
-                const double Us = 0.5 * (at(uy, Nx, Ny, Nz, j, k, l, 1, 0) + at(uy, Nx, Ny, Nz, j, k, l, 1, 1));
                 const double sn = Un >= 0.0 ? face(p, x, j, wrap(k - 1, Ny), l, 1, 0, scheme) : face(p, x, j, k, l, 1, 1, scheme);
                 const double ss = Us >= 0.0 ? face(p, x, j, k, l, 1, 0, scheme) : face(p, x, j, wrap(k + 1, Ny), l, 1, 1, scheme);
-                const double Ur = 0.5 * (at(uz, Nx, Ny, Nz, j, k, l, 2, -1) + at(uz, Nx, Ny, Nz, j, k, l, 2, 0));
-                const double Uf = 0.5 * (at(uz, Nx, Ny, Nz, j, k, l, 2, 0) + at(uz, Nx, Ny, Nz, j, k, l, 2, 1));
                 const double sr = Ur >= 0.0 ? face(p, x, j, k, wrap(l - 1, Nz), 2, 0, scheme) : face(p, x, j, k, l, 2, 1, scheme);
                 const double sf = Uf >= 0.0 ? face(p, x, j, k, l, 2, 0, scheme) : face(p, x, j, k, wrap(l + 1, Nz), 2, 1, scheme);
-                const long id = IDX3(j, k, l, Ny, Nz);
                 out[id] = ux[id] * (se - sw) / dx + uy[id] * (ss - sn) / dy + uz[id] * (sf - sr) / dz;
             }
 }
@@ -739,11 +786,12 @@ void or_rk3_ssp_scalar(const or_params *p, double *ux, double *uy, double *uz,
     for (int c = 0; c < 3; c++) memcpy(u1 + 2 * c * nc, u[c], sizeof(cx) * (size_t)nc);
     or_single_time_step(p, u1, u1 + 2 * nc, u1 + 4 * nc, fx, fy, fz);
     const double *w1[3] = {u1, u1 + 2 * nc, u1 + 4 * nc};
-    scalar_stage(p, scheme, w1, th1, theta, 3.0 / 4.0, 1.0 / 4.0, th2);
+    if (or_mut == 6) scalar_stage(p, scheme, or_mut == 5 ? w0 : w1, th1, theta, 1.0 / 4.0, 3.0 / 4.0, th2);
+    else scalar_stage(p, scheme, or_mut == 5 ? w0 : w1, th1, theta, 3.0 / 4.0, 1.0 / 4.0, th2);
     or_single_time_step(p, u1, u1 + 2 * nc, u1 + 4 * nc, fx, fy, fz);
     for (int c = 0; c < 3; c++) axpby(nc, 3.0 / 4.0, u[c], 1.0 / 4.0, u1 + 2 * c * nc, u2 + 2 * c * nc);
     const double *w2[3] = {u2, u2 + 2 * nc, u2 + 4 * nc};
-    scalar_stage(p, scheme, w2, th2, theta, 1.0 / 3.0, 2.0 / 3.0, th1);
+    scalar_stage(p, scheme, or_mut == 5 ? w0 : w2, th2, theta, 1.0 / 3.0, 2.0 / 3.0, th1);
     or_single_time_step(p, u2, u2 + 2 * nc, u2 + 4 * nc, fx, fy, fz);
     for (int c = 0; c < 3; c++) axpby(nc, 1.0 / 3.0, u[c], 2.0 / 3.0, u2 + 2 * c * nc, u[c]);
     memcpy(theta, th1, sizeof(double) * (size_t)nr);
@@ -876,5 +924,55 @@ void or_jv_reduced(const or_params *p, const double *ux, const double *uy, const
     or_unpack_reduced(p, vec_in, v, v + 2 * nc, v + 4 * nc);
     or_rk3_ssp_uv_rhs(p, ux, uy, uz, v, v + 2 * nc, v + 4 * nc);
     or_pack_reduced(p, v, v + 2 * nc, v + 4 * nc, vec_out);
+    free(v);
+}
+
+/* velocities_from_A_vector_kernel (Source/NS3D_periodic.cu:111-125): full-length
+ * vectors, v_c[i] = i * vec[c*N + i] for every i < N. */
+void or_unpack_full(const or_params *p, const double *vec, double *vx, double *vy, double *vz)
+{
+    const long N = ncomplex(p);
+    double *v[3] = {vx, vy, vz};
+    for (int c = 0; c < 3; c++)
+        for (long i = 0; i < N; i++) {
+            v[c][2 * i] = 0.0;
+            v[c][2 * i + 1] = vec[c * N + i];
+        }
+}
+
+/* A_vector_from_velocities_kernel (:223-236): vec[c*N + i] = Im v_c[i] for
+ * 0 < i < N; the three i = 0 entries are left untouched. */
+void or_pack_full(const or_params *p, const double *vx, const double *vy, const double *vz, double *vec)
+{
+    const long N = ncomplex(p);
+    const double *v[3] = {vx, vy, vz};
+    for (int c = 0; c < 3; c++)
+        for (long i = 1; i < N; i++) vec[c * N + i] = v[c][2 * i + 1];
+}
+
+/* NSCallMatrixVector_reduced_Newton_Globalization (:364-389): the reduced J.v
+ * followed by A_vector_add_diagonal_kernel (:296-310), out -= vec_in / tau. */
+void or_jv_reduced_newton(const or_params *p, const double *ux, const double *uy, const double *uz,
+                          double tau, const double *vec_in, double *vec_out)
+{
+    const long n = 3 * (ncomplex(p) - 1);
+    or_jv_reduced(p, ux, uy, uz, vec_in, vec_out);
+    const double one_over_tau = 1.0 / tau;
+    for (long i = 0; i < n; i++) vec_out[i] = vec_out[i] - vec_in[i] * one_over_tau;
+}
+
+/* NSCallMatrixVector_exponential (:444-470): unpack the full-length vector,
+ * take `timesteps_period` tangent-map steps RK3_SSP_UV (which advance the base
+ * flow u too, in place), pack back with index 0 of each component untouched. */
+void or_jv_exponential(const or_params *p, double *ux, double *uy, double *uz,
+                       const double *fx, const double *fy, const double *fz, int timesteps_period,
+                       const double *vec_in, double *vec_out)
+{
+    const long nc = ncomplex(p);
+    double *v = (double *)malloc(sizeof(cx) * (size_t)nc * 3);
+    or_unpack_full(p, vec_in, v, v + 2 * nc, v + 4 * nc);
+    for (int t = 0; t < timesteps_period; t++)
+        or_rk3_ssp_uv(p, ux, uy, uz, v, v + 2 * nc, v + 4 * nc, fx, fy, fz);
+    or_pack_full(p, v, v + 2 * nc, v + 4 * nc, vec_out);
     free(v);
 }

@@ -105,6 +105,16 @@ void or_unpack_reduced(const or_params *p, const double *vec, double *vx, double
 void or_pack_reduced(const or_params *p, const double *vx, const double *vy, const double *vz, double *vec);
 void or_jv_reduced(const or_params *p, const double *ux, const double *uy, const double *uz,
                    const double *vec_in, double *vec_out);                             /* NSCallMatrixVector_reduced :340-361 */
+void or_unpack_full(const or_params *p, const double *vec, double *vx, double *vy, double *vz);       /* :111-125 */
+void or_pack_full(const or_params *p, const double *vx, const double *vy, const double *vz, double *vec); /* :223-236 */
+void or_jv_reduced_newton(const or_params *p, const double *ux, const double *uy, const double *uz,
+                          double tau, const double *vec_in, double *vec_out);          /* :364-389 */
+void or_jv_exponential(const or_params *p, double *ux, double *uy, double *uz,
+                       const double *fx, const double *fy, const double *fz, int timesteps_period,
+                       const double *vec_in, double *vec_out);                         /* :444-470 */
+
+/* Test-only fault injection: plant mistake m (see ns3d_oracle.c), 0 = none. */
+void or_set_mutation(int m);
 
 #ifdef __cplusplus
 }

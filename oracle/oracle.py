@@ -50,6 +50,22 @@ def lib():
     return _lib
 
 
+class mutation:
+    """Context manager planting mistake m in the oracle (test-only fault
+    injection, see or_set_mutation in ns3d_oracle.c); restores 0 on exit."""
+
+    def __init__(self, m):
+        self.m = int(m)
+
+    def __enter__(self):
+        lib().or_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_mutation(0)
+        return False
+
+
 def _p(a):
     if a is None:
         return None
@@ -257,3 +273,36 @@ class Oracle:
         out = np.empty_like(vec)
         lib().or_jv_reduced(ctypes.byref(self.params), *map(_p, u), _p(vec), _p(out))
         return out
+
+    def pack_full(self, v, vec=None):
+        """A_vector_from_velocities_kernel: writes vec[c*N + i] for 0 < i < N into vec (a copy
+        of the given one, or zeros); the i = 0 entries keep their value."""
+        v = [self._c(c) for c in v]
+        N = self.Nx * self.Ny * self.Mz
+        out = np.zeros(3 * N) if vec is None else np.array(vec, dtype=np.float64, copy=True)
+        lib().or_pack_full(ctypes.byref(self.params), *map(_p, v), _p(out))
+        return out
+
+    def unpack_full(self, vec):
+        vec = np.ascontiguousarray(vec, dtype=np.float64)
+        assert vec.size == 3 * self.Nx * self.Ny * self.Mz
+        v = [np.empty(self.cshape, np.complex128) for _ in range(3)]
+        lib().or_unpack_full(ctypes.byref(self.params), _p(vec), *map(_p, v))
+        return v
+
+    def jv_reduced_newton(self, u, vec, tau):
+        u = [self._c(c) for c in u]
+        vec = np.ascontiguousarray(vec, dtype=np.float64)
+        out = np.empty_like(vec)
+        lib().or_jv_reduced_newton(ctypes.byref(self.params), *map(_p, u), ctypes.c_double(tau), _p(vec), _p(out))
+        return out
+
+    def jv_exponential(self, u, f, timesteps_period, vec, vec_out):
+        """Returns (vec_out after the call, the base flow advanced by timesteps_period steps)."""
+        u = [self._c(c).copy() for c in u]
+        f = [self._c(c) for c in f]
+        vec = np.ascontiguousarray(vec, dtype=np.float64)
+        out = np.array(vec_out, dtype=np.float64, copy=True)
+        lib().or_jv_exponential(ctypes.byref(self.params), *map(_p, u), *map(_p, f), int(timesteps_period),
+                                _p(vec), _p(out))
+        return out, u

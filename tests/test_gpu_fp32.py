@@ -25,8 +25,12 @@ def _rel(a, b):
 
 # fp32s rounds the state to FP32 every step (eps*|u| per step instead of
 # eps*dt*|L(u)|): measured 1.3e-5 after 1000 steps, above the 1e-5 bar that the
-# FP64-state fp32 path meets; its test pins that it stays within 2e-5.
-@pytest.mark.parametrize("precision,tol", [("fp32", TOL), ("fp32s", 2e-5)])
+# FP64-state fp32 path meets.  It is held to the same bar and expected to miss it.
+FP32S_MISSES = pytest.mark.xfail(strict=True, reason="fp32s: FP32 state rounding, measured 1.31e-5 > 1e-5 "
+                                                      "after 1000 steps at 256^3")
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL), pytest.param("fp32s", TOL, marks=FP32S_MISSES)])
 def test_fp32_bulk_error_after_1000_steps_256(precision, tol):
     import torch
     from ns3dperiodic_amd import NS3D, fields

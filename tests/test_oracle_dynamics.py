@@ -226,3 +226,49 @@ def test_divergence_closed_form(oracle_mod):
     div = o.divergence([o.fft_r2c(c) for c in u])
     ref = (np.cos(X) - 2 * np.sin(2 * Y) + np.cos(Z) * np.cos(X)) / dt
     np.testing.assert_allclose(o.fft_c2r(div), ref, atol=1e-12)
+
+
+def test_full_and_reduced_packings_agree(oracle_mod):
+    """The full-length packing (Source/NS3D_periodic.cu:111-125, 223-236) and
+    the reduced one (:173-198, 238-250) are written independently: the reduced
+    vector is the full one with each component's index-0 entry removed, and the
+    full pack leaves those three entries untouched."""
+    o = oracle_mod.Oracle(8, 4, 6)
+    N = 8 * 4 * 4
+    rng = np.random.default_rng(8)
+    v = [rng.standard_normal(o.cshape) + 1j * rng.standard_normal(o.cshape) for _ in range(3)]
+    full = o.pack_full(v, np.full(3 * N, 9.5))
+    red = o.pack_reduced(v)
+    for c in range(3):
+        assert full[c * N] == 9.5
+        np.testing.assert_array_equal(full[c * N + 1:(c + 1) * N], red[c * (N - 1):(c + 1) * (N - 1)])
+        np.testing.assert_array_equal(full[c * N + 1:(c + 1) * N], v[c].ravel()[1:].imag)
+    back = o.unpack_full(full)
+    for c in range(3):
+        assert np.all(back[c].real == 0)
+        np.testing.assert_array_equal(back[c].ravel()[1:].imag, v[c].ravel()[1:].imag)
+
+
+def test_newton_shift_and_exponential_callback(oracle_mod):
+    """NSCallMatrixVector_reduced_Newton_Globalization subtracts vec/tau from
+    the reduced J.v (:364-389); NSCallMatrixVector_exponential (:444-470) is K
+    tangent-map steps between the full-length packings: with K = 1 it equals
+    one RK3_SSP_UV step of the unpacked vector, and it is linear in the vector."""
+    N = 8
+    o = oracle_mod.Oracle(N, dt=4e-3, Re=40.0)
+    u = fields.random_solenoidal(N, seed=21)
+    f = fields.kolmogorov_forcing_sin(N, n=1)
+    rng = np.random.default_rng(22)
+    vec = rng.standard_normal(3 * (N * N * (N // 2 + 1) - 1))
+    tau = 0.3
+    np.testing.assert_allclose(o.jv_reduced_newton(u, vec, tau), o.jv_reduced(u, vec) - vec / tau,
+                               rtol=0, atol=1e-13 * np.abs(vec / tau).max())
+    M = N * N * (N // 2 + 1)
+    w = rng.standard_normal(3 * M)
+    out1, u1 = o.jv_exponential(u, f, 1, w, np.zeros(3 * M))
+    ub, vb = o.rk3_uv(u, o.unpack_full(w), f)
+    assert rel_l2(u1, ub) == 0.0
+    np.testing.assert_array_equal(out1, o.pack_full(vb, np.zeros(3 * M)))
+    out3, _ = o.jv_exponential(u, f, 3, w, np.zeros(3 * M))
+    out3b, _ = o.jv_exponential(u, f, 3, -2.0 * w, np.zeros(3 * M))
+    assert rel_l2(out3b, -2.0 * out3) < 1e-13
