@@ -37,6 +37,7 @@ typedef struct { double x, y; } ns3d_cplx;
 #define NS3D_ERR_HIP (-4)          /* HIP runtime error (launch or copy) */
 #define NS3D_ERR_NONFINITE (-5)    /* NaN/Inf detected by a checking call */
 #define NS3D_ERR_PRECISION (-6)    /* operation not offered at this precision */
+#define NS3D_ERR_CAPACITY (-7)     /* grid too large for ns3d_rk3_ssp_steps_noalloc's fixed scratch */
 
 /* precision of the transform / product bulk; state arrays at the interface are always FP64 */
 #define NS3D_FP64 0
@@ -130,6 +131,20 @@ int ns3d_rk3_ssp(ns3d_ctx *ctx, ns3d_cplx *ux, ns3d_cplx *uy, ns3d_cplx *uz,
                  const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz);  /* RK3_SSP */
 int ns3d_rk3_ssp_steps(ns3d_ctx *ctx, int nsteps, ns3d_cplx *ux, ns3d_cplx *uy, ns3d_cplx *uz,
                        const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz);
+/* ns3d_rk3_ssp_steps without any device or host allocation inside the call.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The fused passes' scratch and the RK stage
+ * buffers come from a fixed 64 MiB device array, one per GPU, held until the process ends
+ * (its address is looked up by ns3d_create); the context's own workspace is neither used nor
+ * grown.  Grids whose scratch exceeds 64 MiB return NS3D_ERR_CAPACITY before any launch:
+ * 15 bulk fields + 6 FP64 stage fields (9 FP32 fields in NS3D_FP32_STATE) of padded rows,
+ * i.e. every grid up to 64^3 in every precision (64^3 FP64: 52.5 MiB); 96^3 does not fit.
+ * Results are bitwise those of ns3d_rk3_ssp_steps.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (two streams, two threads) race and must be ordered by
+ * the caller.  Nothing in the array needs to start at zero: every element a call reads was
+ * written earlier in the same call.  Per-pass profiling is off during the call. */
+int ns3d_rk3_ssp_steps_noalloc(ns3d_ctx *ctx, int nsteps, ns3d_cplx *ux, ns3d_cplx *uy, ns3d_cplx *uz,
+                               const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz);
 
 /* ---- a8: divergence (divergence.cu:8-47) ---- */
 int ns3d_divergence(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz,

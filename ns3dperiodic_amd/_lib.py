@@ -18,6 +18,7 @@ NS3D_FLAG_WENO_ZFIX = 0x2
 NS3D_FLAG_MASK_NYQUIST = 0x4
 NS3D_FLAG_CHECK_FINITE = 0x8
 NS3D_ERR_NONFINITE = -5
+NS3D_ERR_CAPACITY = -7
 UPD_STAGE, UPD_UV_RHS, UPD_UV, UPD_IUV, UPD_RHS = 0, 1, 2, 3, 4
 
 
@@ -60,6 +61,7 @@ SIGNATURES = {
     "ns3d_rhs": (I, [P] + [P] * 9),
     "ns3d_rk3_ssp": (I, [P] + [P] * 6),
     "ns3d_rk3_ssp_steps": (I, [P, I] + [P] * 6),
+    "ns3d_rk3_ssp_steps_noalloc": (I, [P, I] + [P] * 6),
     "ns3d_rk3_ssp_scalar": (I, [P, I] + [P] * 7 + [I]),
     "ns3d_rk3_ssp_graph": (I, [P, I] + [P] * 6),
     "ns3d_rk3_ssp_batch": (I, [P, I, I] + [P] * 4),

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -152,6 +153,12 @@ constexpr int PZ = (PZT<TB, N> / ZPlanT<TB, N>::TPP) > 0 ? (PZT<TB, N> / ZPlanT<
     } while (0)
 
 }  // namespace
+
+// Scratch of the allocation-free entry point ns3d_rk3_ssp_steps_noalloc: one
+// copy per GPU, held until the process ends.  In host code `scratch` is only a
+// symbol for hipGetSymbolAddress, never an address (see scratch_ptr).
+constexpr size_t NS3D_NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NS3D_NOALLOC_SCRATCH_BYTES];
 
 struct ns3d_ctx {
     int Nx, Ny, Nz, Mz;
@@ -927,6 +934,54 @@ int checked(ns3d_ctx* c, int st) {
     return finite_status(c, true);
 }
 
+// ---- allocation-free RK3 steps (ns3d_rk3_ssp_steps_noalloc)
+// The fused passes' scratch (15 bulk fields) and the RK stage buffers (6 FP64
+// fields, or the 9 FP32 state/stage fields of NS3D_FP32_STATE) are carved out
+// of the file-scope device array `scratch` instead of the context's own buffers.
+// Host code never passes the array's name to a kernel: scratch_ptr[dev] is the
+// device address of device dev's copy, looked up once per device by
+// ns3d_create with that device current.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t noalloc_scratch_bytes(const ns3d_ctx* c) {
+    const size_t passes = round256(bulk_bytes(c) * (size_t)c->nci * 15);
+    if (c->precision == NS3D_FP32_STATE) return passes + sizeof(cplx<float>) * (size_t)c->nc16 * 9;
+    return passes + sizeof(zd) * (size_t)c->nc8 * 6;
+}
+
+// The context's scratch, RK stage and FP32 state pointers aimed at the static
+// array for the duration of one call (restored on exit); per-pass timing, which
+// creates events, is off meanwhile.  The fused passes then find every buffer
+// already present and allocate nothing.
+struct ScratchView {
+    ns3d_ctx* c;
+    void* scratch_;
+    int fields_;
+    zd* stage_;
+    cplx<float>* s32_;
+    bool prof_;
+    ScratchView(ns3d_ctx* cc, unsigned char* base)
+        : c(cc), scratch_(cc->scratch), fields_(cc->scratch_fields), stage_(cc->stage), s32_(cc->s32), prof_(cc->prof_on) {
+        unsigned char* rk = base + round256(bulk_bytes(c) * (size_t)c->nci * 15);
+        c->scratch = base;
+        c->scratch_fields = 15;
+        c->stage = (zd*)rk;
+        c->s32 = (cplx<float>*)rk;
+        c->prof_on = false;
+    }
+    ~ScratchView() {
+        c->scratch = scratch_;
+        c->scratch_fields = fields_;
+        c->stage = stage_;
+        c->s32 = s32_;
+        c->prof_on = prof_;
+    }
+};
+
 int ensure_kv(ns3d_ctx* c) {
     if (c->kv_partial) return NS3D_OK;
     if (hipMalloc((void**)&c->kv_partial, sizeof(double) * KV_COLS * KV_BLOCKS) != hipSuccess ||
@@ -952,6 +1007,7 @@ const char* ns3d_status_string(int s) {
         case NS3D_ERR_HIP: return "HIP runtime error";
         case NS3D_ERR_NONFINITE: return "non-finite value";
         case NS3D_ERR_PRECISION: return "operation not offered at this precision";
+        case NS3D_ERR_CAPACITY: return "grid too large for the fixed scratch of the allocation-free entry point";
         default: return "unknown status";
     }
 }
@@ -999,6 +1055,18 @@ int ns3d_create(const ns3d_config* cfg, ns3d_ctx** out) {
     if (!st) {
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->nf, 0);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) st = NS3D_ERR_HIP;
+    }
+    if (!st) {
+        // once per device, with the device current: the address of this GPU's
+        // copy of the allocation-free entry point's scratch
+        std::lock_guard<std::mutex> lock(scratch_mu);
+        if (c->device >= 0 && c->device < NOALLOC_MAX_DEVICES && !scratch_ptr[c->device]) {
+            void* p = nullptr;
+            if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess && p)
+                scratch_ptr[c->device] = p;
+            else
+                (void)hipGetLastError();  // the noalloc entry point then reports NS3D_ERR_HIP
+        }
     }
     if (st) {
         ns3d_destroy(c);
@@ -1250,6 +1318,26 @@ int ns3d_rk3_ssp_steps(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns
     zd* u[3] = {(zd*)ux, (zd*)uy, (zd*)uz};
     const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
     return checked(c, rk3_steps(c, nsteps, u, f));
+}
+
+int ns3d_rk3_ssp_steps_noalloc(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns3d_cplx* uz,
+                               const ns3d_cplx* fx, const ns3d_cplx* fy, const ns3d_cplx* fz) {
+    if (!c || !ux || !uy || !uz || nsteps < 0) return NS3D_ERR_ARG;
+    if (noalloc_scratch_bytes(c) > NS3D_NOALLOC_SCRATCH_BYTES) return NS3D_ERR_CAPACITY;
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(scratch_mu);
+        if (c->device >= 0 && c->device < NOALLOC_MAX_DEVICES) base = scratch_ptr[c->device];
+    }
+    if (!base) return NS3D_ERR_HIP;
+    zd* u[3] = {(zd*)ux, (zd*)uy, (zd*)uz};
+    const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
+    int st;
+    {
+        ScratchView view(c, (unsigned char*)base);
+        st = rk3_steps(c, nsteps, u, f);
+    }
+    return checked(c, st);
 }
 
 int ns3d_rk3_ssp_batch(ns3d_ctx* c, int nbatch, int nsteps, ns3d_cplx* u, const ns3d_cplx* fx, const ns3d_cplx* fy,
