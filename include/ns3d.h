@@ -44,6 +44,9 @@ typedef struct { double x, y; } ns3d_cplx;
 #define NS3D_FP32_BULK 1   /* FP32 transforms and products; state and epilogue FP64 */
 #define NS3D_FP32_STATE 2  /* as FP32_BULK, and the RK3 loops (ns3d_rk3_ssp*, graph) keep the state and
                               stages in FP32, converting the caller's FP64 state once per call */
+#define NS3D_FP32_INCR 3   /* as FP32_BULK, and the RK3 loops carry the SSP stages as FP32 increments over
+                              the FP64 state u^n (u1 = u^n + d1, u2 = u^n + d2) and feed the transforms an
+                              FP32 copy of u^n: FP32 stage storage, FP64 state accuracy */
 
 /* behaviour flags (reference quirks are ON in reference mode) */
 #define NS3D_FLAG_NO_IPQ 0x1u      /* disable zeroing Re(Q^) and Q^(0) (advection_2_3.cu:35-70) */
@@ -137,7 +140,7 @@ int ns3d_rk3_ssp_steps(ns3d_ctx *ctx, int nsteps, ns3d_cplx *ux, ns3d_cplx *uy, 
  * buffers come from a fixed 64 MiB device array, one per GPU, held until the process ends
  * (its address is looked up by ns3d_create); the context's own workspace is neither used nor
  * grown.  Grids whose scratch exceeds 64 MiB return NS3D_ERR_CAPACITY before any launch:
- * 15 bulk fields + 6 FP64 stage fields (9 FP32 fields in NS3D_FP32_STATE) of padded rows,
+ * 15 bulk fields + 6 FP64 stage fields (9 FP32 fields in NS3D_FP32_STATE / NS3D_FP32_INCR) of padded rows,
  * i.e. every grid up to 64^3 in every precision (64^3 FP64: 52.5 MiB); 96^3 does not fit.
  * Results are bitwise those of ns3d_rk3_ssp_steps.  Every call on one GPU shares that GPU's
  * array: calls that overlap on one GPU (two streams, two threads) race and must be ordered by
@@ -270,6 +273,10 @@ int ns3d_cutoff_filter(ns3d_ctx *ctx, double radius, const ns3d_cplx *ux, const 
  *      all three fields to read one point); out3_host = (ux, uy, uz). ---- */
 int ns3d_probe(ns3d_ctx *ctx, const ns3d_cplx *ux, const ns3d_cplx *uy, const ns3d_cplx *uz, int j, int k, int l,
                double *out3_host);
+/* ns3d_probe for nbatch members of a batch in ns3d_rk3_ssp_batch's layout (member b's component c
+ * at u + (3b + c)*Nx*Ny*Mz): out_host[3b + c] = component c of member b at (j, k, l).  One launch
+ * for the batch (execute_return_map's seed disc, Source/return_map.cu:502-690, advanced together). */
+int ns3d_probe_batch(ns3d_ctx *ctx, int nbatch, const ns3d_cplx *u, int j, int k, int l, double *out_host);
 
 /* ---- measurement: per-pass device time of the fused pipeline, recorded with
  *      HIP events on the context stream while enabled.  ms[5]/calls[5] are

@@ -13,6 +13,7 @@ NS3D_OK = 0
 NS3D_FP64 = 0
 NS3D_FP32_BULK = 1
 NS3D_FP32_STATE = 2
+NS3D_FP32_INCR = 3
 NS3D_FLAG_NO_IPQ = 0x1
 NS3D_FLAG_WENO_ZFIX = 0x2
 NS3D_FLAG_MASK_NYQUIST = 0x4
@@ -97,6 +98,7 @@ SIGNATURES = {
     "ns3d_helmholtz_filter": (I, [P, D] + [P] * 6),
     "ns3d_cutoff_filter": (I, [P, D] + [P] * 6),
     "ns3d_probe": (I, [P, P, P, P, I, I, I, DP]),
+    "ns3d_probe_batch": (I, [P, I, P, I, I, I, DP]),
     "ns3d_kinetic_energy": (I, [P, P, P, P, DP]),
     "ns3d_dissipation": (I, [P, P, P, P, DP]),
     "ns3d_cfl_dt": (I, [P, D, P, P, P, DP]),

@@ -214,7 +214,23 @@ template <int N, int PAD, int G> struct PencilMajorSw {
     }
 };
 
-template <typename T, int N, int TPP, int DIR, typename S, typename Addr>
+// Ordering of an LDS exchange.  WAVE = false: a workgroup barrier.  WAVE =
+// true, for layouts where each pencil's LDS region is touched only by that
+// pencil's threads and a pencil's threads lie in one wavefront: LDS operations
+// of one wavefront are performed in program order, so the exchange needs no
+// hardware barrier, only a wavefront-scope fence that keeps the compiler from
+// moving LDS accesses across it; the other waves of the block run on.
+template <bool WAVE> __device__ __forceinline__ void exch_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+template <typename T, int N, int TPP, int DIR, typename S, typename Addr, bool WAVE = false>
 struct PencilFFT {
     using Info = SeqInfo<S>;
     static constexpr int NP = Info::n;
@@ -294,9 +310,9 @@ struct PencilFFT {
         butterflies<I>(v, tw, t);
         if constexpr (I + 1 < NP) {
             exch_write<I>(v, lds, t, p);
-            __syncthreads();
+            exch_sync<WAVE>();
             exch_read<I>(v, lds, t, p);
-            __syncthreads();
+            exch_sync<WAVE>();
             pass<I + 1>(v, lds, tw, t, p);
         }
     }
@@ -310,10 +326,10 @@ struct PencilFFT {
         if constexpr (I + 1 < NP) {
             exch_write<I>(va, la, t, p);
             exch_write<I>(vb, lb, t, p);
-            __syncthreads();
+            exch_sync<WAVE>();
             exch_read<I>(va, la, t, p);
             exch_read<I>(vb, lb, t, p);
-            __syncthreads();
+            exch_sync<WAVE>();
             pass2<I + 1>(va, vb, la, lb, tw, t, p);
         }
     }

@@ -61,10 +61,11 @@ namespace ns3d {
 // w_l = 1 on the l = 0 and l = Nz/2 planes and 2 elsewhere (the C2R convention of
 // the library's transforms).  One pass over the three fields replaces three
 // inverse 3D transforms (return_vector3_solution, Source/return_map.cu:110-160).
-// partial[c*gridDim.x + b], c < 3.
+// blockIdx.y: member of a batch (fields bs elements apart); partial[(3*member
+// + c)*gridDim.x + b], c < 3.
 __global__ void __launch_bounds__(KV_THREADS)
 k_probe_partial(PtrArr<const cplx<double>, 3> u, long nc, int Nx, int Ny, int Nz, int Mz, int jp, int kp, int lp,
-                double* __restrict__ partial) {
+                double* __restrict__ partial, long bs = 0) {
     __shared__ double sh[KV_THREADS / 64];
     double acc[3] = {0.0, 0.0, 0.0};
     const long stride = (long)gridDim.x * KV_THREADS;
@@ -78,14 +79,14 @@ k_probe_partial(PtrArr<const cplx<double>, 3> u, long nc, int Nx, int Ny, int Nz
         const double w = (l == 0 || 2 * l == Nz) ? 1.0 : 2.0;
 #pragma unroll
         for (int f = 0; f < 3; f++) {
-            const cplx<double> v = u.p[f][g];
+            const cplx<double> v = u.p[f][(long)blockIdx.y * bs + g];
             acc[f] += w * (v.x * co - v.y * s);
         }
     }
 #pragma unroll
     for (int f = 0; f < 3; f++) {
         const double r = kv_block_sum(acc[f], sh);
-        if (threadIdx.x == 0) partial[(long)f * gridDim.x + blockIdx.x] = r;
+        if (threadIdx.x == 0) partial[((long)blockIdx.y * 3 + f) * gridDim.x + blockIdx.x] = r;
     }
 }
 

@@ -53,10 +53,13 @@ template <int N, int E> using XYPlan = typename XYSel<N, E>::type;
 // grid.y enumerates the fields; with OS = 1 each block produces one output
 // (w_b or dx w_b) and the input is read by two blocks.
 // TS: scalar type of the state w (double, or float for the FP32-state RK3 loop).
-template <typename TB, int NX, int P, int NF, int E, int OS, typename TS = double>
+// ADD = 1: the transformed field is w + w2 (w2 of the same type and layout; the
+// FP32-increment RK3 loop's stage value base + increment), summed in FP64.
+template <typename TB, int NX, int P, int NF, int E, int OS, typename TS = double, int ADD = 0>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
 k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int Mw, int Ms,
-             int nlt, int nb, SpecTables tab, const cplx<TB>* __restrict__ twg, long bwi = 0, long bsc = 0) {
+             int nlt, int nb, SpecTables tab, const cplx<TB>* __restrict__ twg, long bwi = 0, long bsc = 0,
+             PtrArr<const cplx<TS>, 3> w2 = {}) {
     constexpr int TPP = XYPlan<NX, E>::TPP;
     using F = PencilFFT<TB, NX, TPP, +1, typename XYPlan<NX, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NX * P];
@@ -78,6 +81,7 @@ k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc
     const int f = OS ? (int)blockIdx.y >> 1 : (int)blockIdx.y;
     const int which = OS ? (int)blockIdx.y & 1 : 0;  // OS: 0 -> w_b, 1 -> dx w_b
     const cplx<TS>* __restrict__ src = w.p[f] + (long)blockIdx.z * bwi;
+    const cplx<TS>* __restrict__ src2 = ADD ? w2.p[f % 3] : nullptr;
     A += (long)blockIdx.z * bsc;
     cplx<TB> v[F::E], d[OS ? 1 : F::E];
 #pragma unroll
@@ -89,7 +93,10 @@ k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc
             double kx = 0.0;
             if (F::in_active(t, q)) {
                 kx = tab.kx[j];
-                if (keep && mask_keep(tab, j, k, l)) s = cvt<double>(src[base + j * S]);
+                if (keep && mask_keep(tab, j, k, l)) {
+                    s = cvt<double>(src[base + j * S]);
+                    if constexpr (ADD != 0) s = s + cvt<double>(src2[base + j * S]);
+                }
             }
             const cplx<TB> dv = mk<TB>((TB)(-s.y * kx), (TB)(s.x * kx));
             if constexpr (OS) {
@@ -258,8 +265,15 @@ struct ZPencil {
 #endif
     static constexpr int SWG = sizeof(TB) == 8 ? (NZ % 3 == 0 && NZ >= 192 ? NS3D_R3_ZSWZ : NS3D_ZSWZ) : NS3D_ZSWZ32;
     using LA = typename std::conditional<SWG == 0, PencilMajor<NZ, PAD>, PencilMajorSw<NZ, PAD, (SWG == 0 ? 1 : SWG)>>::type;
-    using FI = PencilFFT<TB, NZ, TPP, +1, typename ZPlanT<TB, NZ>::R, LA>;
-    using FF = PencilFFT<TB, NZ, TPP, -1, typename SeqRev<typename ZPlanT<TB, NZ>::R>::type, LA>;
+    // Each pencil's LDS region is private to its TPP threads; when TPP divides
+    // the wavefront size, those threads lie in one wavefront and every exchange
+    // is ordered within the wave (exch_sync<true>) instead of by a block barrier.
+#ifndef NS3D_ZWAVE
+#define NS3D_ZWAVE 1
+#endif
+    static constexpr bool WAVE = NS3D_ZWAVE != 0 && TPP <= 64 && 64 % TPP == 0;
+    using FI = PencilFFT<TB, NZ, TPP, +1, typename ZPlanT<TB, NZ>::R, LA, WAVE>;
+    using FF = PencilFFT<TB, NZ, TPP, -1, typename SeqRev<typename ZPlanT<TB, NZ>::R>::type, LA, WAVE>;
     static_assert(FI::RL == FF::R0 && FI::BL == FF::B0, "mirror plans must share the product layout");
     static constexpr int E = FI::E > FF::E ? FI::E : FF::E;
     static constexpr int NPT = FI::BL * FI::RL;  // physical points per thread
@@ -392,7 +406,7 @@ struct ZPencil {
                 lb[LA::at(p, FF::out_index(t, q, r))] = vb[q * FF::RL + r];
             }
         }
-        __syncthreads();
+        exch_sync<WAVE>();
         if (valid) {
             for (int n = t; n <= NZ / 2; n += TPP) {
                 const cplx<TB> zn = la[LA::at(p, n)];
@@ -402,7 +416,7 @@ struct ZPencil {
                 o3[n] = lb[LA::at(p, n)];
             }
         }
-        __syncthreads();
+        exch_sync<WAVE>();
     }
 
     // Forward-transform (q1 + i q2) and store the two half spectra (q2 may be null).
@@ -419,7 +433,7 @@ struct ZPencil {
 #pragma unroll
             for (int r = 0; r < FF::RL; r++) lds[LA::at(p, FF::out_index(t, q, r))] = v[q * FF::RL + r];
         }
-        __syncthreads();
+        exch_sync<WAVE>();
         if (valid) {
             for (int n = t; n <= NZ / 2; n += TPP) {
                 const cplx<TB> zn = lds[LA::at(p, n)];
@@ -432,7 +446,7 @@ struct ZPencil {
                 }
             }
         }
-        __syncthreads();
+        exch_sync<WAVE>();
     }
 };
 
@@ -671,7 +685,19 @@ k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, i
 // ---------------------------------------------------------------- P5 epilogues
 // EPI_STAGE32: EPI_STAGE with w, u0 and out in FP32 (the FP32-state RK3 loop);
 // the arithmetic and the forcing stay FP64.
-enum EpiMode { EPI_Q = 0, EPI_STAGE = 1, EPI_RHS = 2, EPI_UV = 3, EPI_UV_RHS = 4, EPI_IUV = 5, EPI_STAGE32 = 6 };
+// EPI_INC / EPI_INC_FINAL: the SSP stages in increment form over an FP64 base
+// state (the FP32-increment RK3 loop).  With w = base + d (d: the previous
+// stage's FP32 increment, u0; null in the first stage):
+//   EPI_INC        out   = a1*(d + dt*L(w))          base FP32 (w), out FP32
+//   EPI_INC_FINAL  out   = base + a1*(d + dt*L(w))   base FP64 (w), out FP64,
+//                  out32 = the FP32 copy of out
+// which is RK3_SSP's u1 = u + dt L(u), u2 = 3/4 u + 1/4 (u1 + dt L(u1)),
+// u = 1/3 u + 2/3 (u2 + dt L(u2)) with a1 = 1, 1/4, 2/3 and the stage values
+// u1 = u + d1, u2 = u + d2.
+enum EpiMode {
+    EPI_Q = 0, EPI_STAGE = 1, EPI_RHS = 2, EPI_UV = 3, EPI_UV_RHS = 4, EPI_IUV = 5, EPI_STAGE32 = 6,
+    EPI_INC = 7, EPI_INC_FINAL = 8
+};
 
 struct EpiArgs {
     cplx<double>* w[3];          // state (read; written in place for UV/UV_RHS/IUV); cplx<float> for EPI_STAGE32
@@ -687,6 +713,9 @@ struct EpiArgs {
     long bw, bu0, bo;
     // sticky non-finite flag: set to 1 by any lane that writes a NaN or Inf (may be null)
     int* nf;
+    // EPI_INC_FINAL: FP32 copy of the new state, rows of stride m32
+    cplx<float>* out32[3];
+    int m32;
 };
 
 // Non-finite flag of the fused epilogues and the scalar stage: every lane that
@@ -736,6 +765,29 @@ __device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& ta
             const cplx<float> rf = cvt<float>(r);
             reinterpret_cast<cplx<float>*>(ea.out[b])[go] = rf;
             bad |= !finite2(rf);
+            continue;
+        }
+        if (MODE == EPI_INC || MODE == EPI_INC_FINAL) {
+            const cplx<double> base =
+                MODE == EPI_INC_FINAL ? ea.w[b][gw] : cvt<double>(reinterpret_cast<const cplx<float>*>(ea.w[b])[gw]);
+            const cplx<double> d =
+                ea.u0[b] ? cvt<double>(reinterpret_cast<const cplx<float>*>(ea.u0[b])[gu]) : mk<double>(0.0, 0.0);
+            cplx<double> f = mk<double>(0.0, 0.0);
+            if (ea.f[b]) f = ea.f[b][gf];
+            const double Gd = c.din * dt / Re;
+            r.x = ea.a1 * (d.x + (dt * AQr[b] - Gd * (base.x + d.x) + dt * f.x - dt * Q[b].x));
+            r.y = ea.a1 * (d.y + (dt * AQi[b] - Gd * (base.y + d.y) + dt * f.y - dt * Q[b].y));
+            if (MODE == EPI_INC) {
+                const cplx<float> rf = cvt<float>(r);
+                reinterpret_cast<cplx<float>*>(ea.out[b])[go] = rf;
+                bad |= !finite2(rf);
+            } else {
+                r.x += base.x;
+                r.y += base.y;
+                ea.out[b][go] = r;
+                ea.out32[b][row * ea.m32 + l] = cvt<float>(r);
+                bad |= !finite2(r);
+            }
             continue;
         }
         const cplx<double> w = ea.w[b][gw];

@@ -226,6 +226,10 @@ struct ns3d_ctx {
     // sticky non-finite flag written by the fused epilogues and the scalar stage
     int* nf = nullptr;
     int* nf_host = nullptr;  // pinned
+    // batched probe: partial sums and results (device), results (pinned host)
+    double* pb = nullptr;
+    double* pb_host = nullptr;
+    size_t pb_cap = 0;
 };
 
 namespace {
@@ -428,8 +432,11 @@ inline dim3 xgrid(long nb, int nfy = 1, int nz = 1) {
 // Runs P1..P4 for NF vector fields and leaves D (3 fields) at scratch + 3*nci.
 // mw: row stride of the input fields (0: Mz, the reference layout)
 // bin: batch member stride of the inputs (c->batch members, scratch c->bsc apart).
+// in2 (nf = 1 only, may be null): a second field of in's type and layout added
+// to the input before the transforms (the FP32-increment loop's base + increment).
 template <typename TB, typename TS = double>
-int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw = 0, long bin = 0) {
+int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw = 0, long bin = 0,
+                         const cplx<TS>* const* in2 = nullptr) {
     if (mw == 0) mw = c->Mz;
     int st = ensure_scratch(c, 15 * nf * c->batch);
     if (st) return st;
@@ -451,7 +458,12 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
         const int nb = c->Ny * nlt;
         constexpr int OS = NS3D_OS_XINV;
         dim3 grid = xgrid(nb, 3 * nf * (OS ? 2 : 1), nz), blk(P * XYPlan<NX, E>::TPP);
-        if (nf == 1) {
+        if (nf == 1 && in2) {
+            PtrArr<const cplx<TS>, 3> w{{in[0], in[1], in[2]}};
+            PtrArr<const cplx<TS>, 3> w2{{in2[0], in2[1], in2[2]}};
+            hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E, OS, TS, 1>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, mw, Mi, nlt, nb,
+                               c->tab, twp<TB>(c, 0), bin, bsc, w2);
+        } else if (nf == 1) {
             PtrArr<const cplx<TS>, 3> w{{in[0], in[1], in[2]}};
             hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E, OS, TS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, mw, Mi, nlt, nb, c->tab,
                                twp<TB>(c, 0), bin, bsc);
@@ -660,6 +672,51 @@ int ensure_state32(ns3d_ctx* c) {
     return NS3D_OK;
 }
 
+// ---- NS3D_FP32_INCR: FP32 transforms and products over an FP64 state.  The
+// RK3 stages are carried as FP32 increments d1, d2 over the FP64 state u^n
+// (u1 = u^n + d1, u2 = u^n + d2; EPI_INC / EPI_INC_FINAL), so the rounding of
+// a stage value is eps32*|d| instead of eps32*|u|; the transforms read an FP32
+// copy of u^n (s32[0..2], rows m16) that the last stage rewrites with u^{n+1}.
+// s32: copy of u^n [3], d1 [3], d2 [3].
+int stage_incr(ns3d_ctx* c, const cf* const d[3], double a1, cf* const out[3], const zd* const f[3],
+               zd* const u_final[3]) {
+    const long nc = c->nc16;
+    const cf* ub[3] = {c->s32, c->s32 + nc, c->s32 + 2 * nc};
+    EpiArgs ea = make_epi(c);
+    for (int b = 0; b < 3; b++) {
+        ea.w[b] = u_final ? u_final[b] : (zd*)ub[b];
+        ea.u0[b] = d ? (const zd*)d[b] : nullptr;
+        ea.f[b] = f ? f[b] : nullptr;
+        ea.out[b] = u_final ? u_final[b] : (zd*)out[b];
+        ea.out32[b] = (cf*)ub[b];
+    }
+    ea.a1 = a1;
+    ea.mw = u_final ? c->Mz : c->m16;
+    ea.mo = u_final ? c->Mz : c->m16;
+    ea.mu0 = c->m16;
+    ea.m32 = c->m16;
+    int st = run_nonlinear_passes<float, float>(c, ub, 1, c->m16, 0, d);
+    if (st) return st;
+    st = u_final ? run_x_forward_epilogue<float, EPI_INC_FINAL>(c, ea) : run_x_forward_epilogue<float, EPI_INC>(c, ea);
+    if (st) return st;
+    return hipGetLastError() == hipSuccess ? NS3D_OK : NS3D_ERR_HIP;
+}
+
+int rk3_one_incr(ns3d_ctx* c, zd* const u[3], const zd* const f[3]) {
+    const long nc = c->nc16;
+    cf* d1[3] = {c->s32 + 3 * nc, c->s32 + 4 * nc, c->s32 + 5 * nc};
+    cf* d2[3] = {c->s32 + 6 * nc, c->s32 + 7 * nc, c->s32 + 8 * nc};
+    const cf* d1c[3] = {d1[0], d1[1], d1[2]};
+    const cf* d2c[3] = {d2[0], d2[1], d2[2]};
+    int st;
+    if ((st = stage_incr(c, nullptr, 1.0, d1, f, nullptr))) return st;
+    if ((st = stage_incr(c, d1c, 1.0 / 4.0, d2, f, nullptr))) return st;
+    return stage_incr(c, d2c, 2.0 / 3.0, nullptr, f, u);
+}
+
+// the RK3 loops of NS3D_FP32_STATE and NS3D_FP32_INCR keep 9 FP32 fields (s32)
+bool uses_s32(const ns3d_ctx* c) { return c->precision == NS3D_FP32_STATE || c->precision == NS3D_FP32_INCR; }
+
 // nsteps RK3 steps in the context's precision
 // One RK3 step of nb same-size fields with one set of launches: member b's
 // component k at u + (3b + k)*nc (reference layout); forcing shared.  The
@@ -700,6 +757,19 @@ int rk3_one_batch(ns3d_ctx* c, int nb, zd* u, const zd* const f[3]) {
 }
 
 int rk3_steps(ns3d_ctx* c, int nsteps, zd* const u[3], const zd* const f[3]) {
+    if (c->precision == NS3D_FP32_INCR) {
+        if (nsteps == 0) return NS3D_OK;
+        int st = ensure_state32(c);
+        if (!st) st = ensure_scratch(c, 15);
+        if (st) return st;
+        for (int b = 0; b < 3; b++)
+            hipLaunchKernelGGL(k_d2f_rows, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, u[b], c->s32 + b * c->nc16, c->nc,
+                               c->Mz, c->m16);
+        HIPCHK(hipGetLastError());
+        for (int i = 0; i < nsteps; i++)
+            if ((st = rk3_one_incr(c, u, f))) return st;
+        return NS3D_OK;
+    }
     if (c->precision != NS3D_FP32_STATE) {
         for (int i = 0; i < nsteps; i++) {
             int st = rk3_one(c, u, f);
@@ -936,7 +1006,7 @@ int checked(ns3d_ctx* c, int st) {
 
 // ---- allocation-free RK3 steps (ns3d_rk3_ssp_steps_noalloc)
 // The fused passes' scratch (15 bulk fields) and the RK stage buffers (6 FP64
-// fields, or the 9 FP32 state/stage fields of NS3D_FP32_STATE) are carved out
+// fields, or the 9 FP32 fields of NS3D_FP32_STATE / NS3D_FP32_INCR) are carved out
 // of the file-scope device array `scratch` instead of the context's own buffers.
 // Host code never passes the array's name to a kernel: scratch_ptr[dev] is the
 // device address of device dev's copy, looked up once per device by
@@ -949,7 +1019,7 @@ size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 
 size_t noalloc_scratch_bytes(const ns3d_ctx* c) {
     const size_t passes = round256(bulk_bytes(c) * (size_t)c->nci * 15);
-    if (c->precision == NS3D_FP32_STATE) return passes + sizeof(cplx<float>) * (size_t)c->nc16 * 9;
+    if (uses_s32(c)) return passes + sizeof(cplx<float>) * (size_t)c->nc16 * 9;
     return passes + sizeof(zd) * (size_t)c->nc8 * 6;
 }
 
@@ -1019,7 +1089,8 @@ int ns3d_create(const ns3d_config* cfg, ns3d_ctx** out) {
     *out = nullptr;
     if (!size_ok(cfg->Nx) || !size_ok(cfg->Ny) || !size_ok(cfg->Nz)) return NS3D_ERR_SIZE;
     if (!(cfg->Lx > 0) || !(cfg->Ly > 0) || !(cfg->Lz > 0) || !(cfg->Re > 0)) return NS3D_ERR_ARG;
-    if (cfg->precision != NS3D_FP64 && cfg->precision != NS3D_FP32_BULK && cfg->precision != NS3D_FP32_STATE)
+    if (cfg->precision != NS3D_FP64 && cfg->precision != NS3D_FP32_BULK && cfg->precision != NS3D_FP32_STATE &&
+        cfg->precision != NS3D_FP32_INCR)
         return NS3D_ERR_ARG;
     if (cfg->device >= 0) HIPCHK(hipSetDevice(cfg->device));
     ns3d_ctx* c = new (std::nothrow) ns3d_ctx();
@@ -1102,6 +1173,8 @@ int ns3d_destroy(ns3d_ctx* c) {
     if (c->red_host) hipHostFree(c->red_host);
     hipFree(c->nf);
     if (c->nf_host) hipHostFree(c->nf_host);
+    hipFree(c->pb);
+    if (c->pb_host) hipHostFree(c->pb_host);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
@@ -1346,8 +1419,8 @@ int ns3d_rk3_ssp_batch(ns3d_ctx* c, int nbatch, int nsteps, ns3d_cplx* u, const 
     if (nbatch > NS3D_MAX_BATCH) return NS3D_ERR_SIZE;
     const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
     zd* ub = (zd*)u;
-    if (c->precision == NS3D_FP32_STATE) {
-        // the FP32 state loop keeps one member's state: members run one after another
+    if (uses_s32(c)) {
+        // the FP32 state / increment loop keeps one member's stages: members run one after another
         for (int b = 0; b < nbatch; b++) {
             zd* m[3] = {ub + 3 * b * c->nc, ub + (3 * b + 1) * c->nc, ub + (3 * b + 2) * c->nc};
             int st = rk3_steps(c, nsteps, m, f);
@@ -1385,7 +1458,7 @@ int ns3d_rk3_ssp_graph(ns3d_ctx* c, int nsteps, ns3d_cplx* ux, ns3d_cplx* uy, ns
         drop_graph(c);
         // every buffer the step uses is allocated before capture
         int st = ensure_scratch(c, 15);
-        if (!st) st = c->precision == NS3D_FP32_STATE ? ensure_state32(c) : ensure_fp64(c, &c->stage, 9);
+        if (!st) st = uses_s32(c) ? ensure_state32(c) : ensure_fp64(c, &c->stage, 9);
         if (st) return st;
         HIPCHK(hipStreamSynchronize(c->stream));
         const bool prof = c->prof_on;
@@ -1825,6 +1898,41 @@ int ns3d_probe(ns3d_ctx* c, const ns3d_cplx* ux, const ns3d_cplx* uy, const ns3d
     HIPCHK(hipStreamSynchronize(c->stream));
     const double inv = 1.0 / (1.0 * c->Nx * c->Ny * c->Nz);
     for (int b = 0; b < 3; b++) out3[b] = c->kv_host[b] * inv;
+    return NS3D_OK;
+}
+
+int ns3d_probe_batch(ns3d_ctx* c, int nbatch, const ns3d_cplx* u, int j, int k, int l, double* out_host) {
+    if (!c || !u || !out_host || nbatch < 1 || nbatch > NS3D_MAX_BATCH || j < 0 || k < 0 || l < 0 || j >= c->Nx ||
+        k >= c->Ny || l >= c->Nz)
+        return NS3D_ERR_ARG;
+    const int nb = (int)std::min<long>(64, (c->nc + KV_THREADS - 1) / KV_THREADS);
+    const size_t need = (size_t)3 * nbatch * (nb + 1);
+    if (c->pb_cap < need) {
+        if (c->pb) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            hipFree(c->pb);
+            hipHostFree(c->pb_host);
+            c->pb = nullptr;
+            c->pb_host = nullptr;
+            c->pb_cap = 0;
+        }
+        if (hipMalloc((void**)&c->pb, sizeof(double) * need) != hipSuccess ||
+            hipHostMalloc((void**)&c->pb_host, sizeof(double) * 3 * nbatch) != hipSuccess)
+            return NS3D_ERR_ALLOC;
+        c->pb_cap = need;
+    }
+    const zd* ub = (const zd*)u;
+    PtrArr<const zd, 3> p{{ub, ub + c->nc, ub + 2 * c->nc}};
+    double* partial = c->pb;
+    double* out = c->pb + (size_t)3 * nbatch * nb;
+    hipLaunchKernelGGL(k_probe_partial, dim3(nb, nbatch), dim3(KV_THREADS), 0, c->stream, p, c->nc, c->Nx, c->Ny, c->Nz,
+                       c->Mz, j, k, l, partial, 3 * c->nc);
+    hipLaunchKernelGGL(k_kv_final, dim3(3 * nbatch), dim3(KV_THREADS), 0, c->stream, partial, nb, out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->pb_host, out, sizeof(double) * 3 * nbatch, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double inv = 1.0 / (1.0 * c->Nx * c->Ny * c->Nz);
+    for (int i = 0; i < 3 * nbatch; i++) out_host[i] = c->pb_host[i] * inv;
     return NS3D_OK;
 }
 

@@ -281,7 +281,8 @@ def _apply_shifts(H, shifts):
     return H, Q
 
 
-def eigs(ops, matvec, n, k=6, m=30, which="LM", tol=1e-10, max_iter=300, seed=0, v0=None, vectors=False, buffer=None):
+def eigs(ops, matvec, n, k=6, m=30, which="LM", tol=1e-10, max_iter=300, seed=0, v0=None, vectors=False, buffer=None,
+         residuals=True):
     """k eigenvalues of the operator `matvec(x, y)` (y = A x, length-n FP64
     vectors) with an implicitly restarted Arnoldi of m vectors.  Each restart
     keeps k + buffer Ritz vectors (default buffer max(2, k/2)) and converges
@@ -291,7 +292,8 @@ def eigs(ops, matvec, n, k=6, m=30, which="LM", tol=1e-10, max_iter=300, seed=0,
 
     Returns dict(values, residuals, iterations, ritz_norm[, vectors_real,
     vectors_imag]): eigenvalues sorted by `which`, residuals ||A x - lambda x||
-    of the unit-norm eigenvectors computed with A, the number of restarts.
+    of the unit-norm eigenvectors computed with A (skipped, NaN, with
+    residuals=False), the number of restarts.
     """
     if not (0 < k < m <= 128 and m <= n):
         raise ValueError("need 0 < k < m <= min(n, 128)")
@@ -351,7 +353,7 @@ def eigs(ops, matvec, n, k=6, m=30, which="LM", tol=1e-10, max_iter=300, seed=0,
     ops.gemm(V, m, Yi, Xi)
     res = []
     ar, ai = ops.zeros(n), ops.zeros(n)
-    for i in range(k):
+    for i in range(k if residuals else 0):
         nr2 = ops.dot(Xr[i], Xr[i]) + ops.dot(Xi[i], Xi[i])
         nrm = math.sqrt(nr2) if nr2 > 0 else 1.0
         matvec(Xr[i], ar)
@@ -363,7 +365,8 @@ def eigs(ops, matvec, n, k=6, m=30, which="LM", tol=1e-10, max_iter=300, seed=0,
         ops.axpby(-a, Xi[i], 1.0, ai)
         ops.axpby(-b, Xr[i], 1.0, ai)
         res.append(math.sqrt(max(ops.dot(ar, ar) + ops.dot(ai, ai), 0.0)) / nrm)
-    out = {"values": lam, "residuals": np.array(res), "iterations": it, "ritz_norm": ritz_norm}
+    out = {"values": lam, "residuals": np.array(res) if residuals else np.full(k, np.nan), "iterations": it,
+           "ritz_norm": ritz_norm}
     if vectors:
         out["vectors_real"], out["vectors_imag"] = Xr, Xi
     return out
@@ -512,6 +515,137 @@ def fd_jv_batch(g, u, V, eps=1e-7, steps=1, f=None, central=False):
     return (ub[1:] - ub[0]) / eps
 
 
+# ----------------------------------------------------------------------------- rotated shift-invert eigen mode
+
+
+def rotated_shift_invert(ops, expo, n, shift, angle, L=6, tol=1e-9, max_iter=None, stats=None):
+    """x -> (E - sigma)^-1 x, sigma = shift * exp(i angle), on the doubled real
+    vector [x_r; x_i] (length 2n) of a real operator E (`expo(x, y)`: y = E x,
+    length n): Axb_rotated_exponent_invert (Source/NS3D_periodic.cu:590-630),
+    whose matvec NSCallMatrixVector_exponential_rotated_reduced (:541-586) is
+        [y_r; y_i] = [E x_r - sr x_r + si x_i ; E x_i - sr x_i - si x_r]
+    (sr, si = shift cos(angle), shift sin(angle)): the real form of the complex
+    operator E - sigma acting on x_r + i x_i.  Each application solves that
+    system by BiCGStab(L) from a zero start and raises if it does not converge
+    (the reference exits).  `stats`, if a list, receives the iterations of
+    every solve.  Returns the matvec of the inverse."""
+    sr, si = shift * math.cos(angle), shift * math.sin(angle)
+    max_iter = 2 * n if max_iter is None else max_iter
+
+    def rot(x, y):
+        xr, xi, yr, yi = x[:n], x[n:], y[:n], y[n:]
+        expo(xr, yr)
+        ops.axpby(-sr, xr, 1.0, yr)
+        ops.axpby(si, xi, 1.0, yr)
+        expo(xi, yi)
+        ops.axpby(-sr, xi, 1.0, yi)
+        ops.axpby(-si, xr, 1.0, yi)
+        return y
+
+    def inv(b, x):
+        ops.axpby(0.0, b, 0.0, x)
+        flag, it, res = bicgstab_l(ops, rot, b, x, L=L, tol=tol, max_iter=max_iter)
+        if flag != 0:
+            raise RuntimeError(f"rotated shift-invert: BiCGStab({L}) flag {flag} after {it} iterations, residual {res:.2e}")
+        if stats is not None:
+            stats.append(it)
+        return x
+
+    return inv
+
+
+def _real_basis(ops, Xr, Xi, k, rank_tol=1e-10):
+    """Orthonormal real basis (rows of an ops basis) of span{Xr_i, Xi_i, i < k}:
+    the Schur basis of the invariant subspace spanned by k Ritz vectors (a
+    complex pair contributes its real and imaginary parts)."""
+    n2 = Xr.shape[-1]
+    Z = ops.zeros(n2, 2 * k)
+    for i in range(k):
+        ops.copy(Xr[i], Z[i])
+        ops.copy(Xi[i], Z[k + i])
+    G = np.zeros((2 * k, 2 * k))
+    for i in range(2 * k):
+        G[i, : i + 1] = ops.gemv_t(Z, i + 1, Z[i])
+    G = G + np.tril(G, -1).T
+    w, U = np.linalg.eigh(G)
+    keep = w > rank_tol * max(w.max(), 1e-300)
+    w, U = w[keep][::-1], U[:, keep][:, ::-1]
+    C = U / np.sqrt(w)                      # Z^T C: orthonormal columns
+    W = ops.zeros(n2, C.shape[1])
+    ops.gemm(Z, 2 * k, C, W)
+    return W, C.shape[1]
+
+
+def eigs_rotated_shift_invert(ops, jv, expo, n, k=6, m=30, shift=1.05, angle=0.05, L=6, lin_tol=1e-9,
+                              lin_iter=None, tol=5e-8, max_iter=3000, which="LR", seed=0):
+    """The reference's live eigen mode: Implicit_restart_Arnoldi_GPU_data_Matrix_Exponent
+    with is_rotated (Source/Arnoldi/Implicit_restart_Arnoldi.cu:456-741), as
+    main calls it (Source/NS3D_periodic.cu:1710: operator Axb_rotated_exponent_invert,
+    "LM" on the inverse, "LR" for the original map).
+
+    1. IRA ("LM", m vectors, k wanted) on (E - sigma)^-1 over doubled vectors
+       of length 2n (``rotated_shift_invert``): the eigenvalues mu of E closest
+       to sigma = shift * exp(i angle).
+    2. An orthonormal real basis W = [W_r; W_i] of the converged invariant
+       subspace (the reference's Schur vectors V(:,1:k) Q).
+    3. compute_eigensystem_of_original_mapping_rotation (:334-430): the
+       Galerkin matrix R = W_r^T J W_r + W_i^T J W_i of the original operator J
+       (`jv(x, y)`: y = J x, length n), eig(R) sorted by `which`, eigenvectors
+       (W_r + W_i) y.
+
+    A real eigenvalue mu of E gives two eigenvalues of the doubled operator
+    (mu - sigma and mu - conj(sigma)), so, as in the reference, each real
+    eigenvalue of J appears twice among `values`; `distinct` merges them.
+
+    Returns dict(values, distinct, residuals ||J x - lambda x|| / ||x||,
+    inverse_values (the Ritz values of the inverse), iterations (restarts),
+    ritz_norm, solves (BiCGStab iterations of every inner solve), basis_rank)."""
+    solves = []
+    inv = rotated_shift_invert(ops, expo, n, shift, angle, L=L, tol=lin_tol, max_iter=lin_iter, stats=solves)
+    v0 = np.random.default_rng(seed).normal(0.0, 2.0, 2 * n)
+    r = eigs(ops, inv, 2 * n, k=k, m=m, which="LM", tol=tol, max_iter=max_iter, v0=v0, vectors=True, residuals=False)
+    W, rank = _real_basis(ops, r["vectors_real"], r["vectors_imag"], k)
+    Wr, Wi = ops.zeros(n, rank), ops.zeros(n, rank)
+    for i in range(rank):
+        ops.copy(W[i][:n], Wr[i])
+        ops.copy(W[i][n:], Wi[i])
+    JW = ops.zeros(n)
+    R = np.zeros((rank, rank))
+    for B in (Wr, Wi):
+        for j in range(rank):
+            jv(B[j], JW)
+            R[:, j] += ops.gemv_t(B, rank, JW)
+    lam, Y = np.linalg.eig(R)
+    sel = _order(lam, which)[:min(k, rank)]
+    lam, Y = lam[sel], Y[:, sel]
+    S = ops.zeros(n, rank)
+    for i in range(rank):
+        ops.copy(Wr[i], S[i])
+        ops.axpby(1.0, Wi[i], 1.0, S[i])
+    kk = len(sel)
+    Xr, Xi = ops.zeros(n, kk), ops.zeros(n, kk)
+    ops.gemm(S, rank, np.ascontiguousarray(Y.real), Xr)
+    ops.gemm(S, rank, np.ascontiguousarray(Y.imag), Xi)
+    res = []
+    ar, ai = ops.zeros(n), ops.zeros(n)
+    for i in range(kk):
+        nrm = math.sqrt(max(ops.dot(Xr[i], Xr[i]) + ops.dot(Xi[i], Xi[i]), 0.0)) or 1.0
+        jv(Xr[i], ar)
+        jv(Xi[i], ai)
+        a, b = lam[i].real, lam[i].imag
+        ops.axpby(-a, Xr[i], 1.0, ar)
+        ops.axpby(b, Xi[i], 1.0, ar)
+        ops.axpby(-a, Xi[i], 1.0, ai)
+        ops.axpby(-b, Xr[i], 1.0, ai)
+        res.append(math.sqrt(max(ops.dot(ar, ar) + ops.dot(ai, ai), 0.0)) / nrm)
+    distinct = []
+    for z in lam:
+        if all(abs(z - d) > 1e-8 * max(1.0, abs(z)) for d in distinct):
+            distinct.append(z)
+    return {"values": lam, "distinct": np.array(distinct), "residuals": np.array(res), "inverse_values": r["values"],
+            "iterations": r["iterations"], "ritz_norm": r["ritz_norm"], "solves": solves, "basis_rank": rank}
+
+
 # ----------------------------------------------------------------------------- NS operators
 
 
@@ -561,6 +695,19 @@ class NSOperators:
     def jacobian(self, x):
         self.set_base(self.g.unpack_reduced(x))
         return self.jv
+
+    def implicit_exponent(self, steps):
+        """v -> E v with E = (single_time_step_iUV_RHS)^steps around the fixed
+        base state u, time step the context's dt: NSCallMatrixVector_exponential_reduced
+        (Source/NS3D_periodic.cu:474-500) without its shift; the operator of
+        the rotated shift-invert eigen mode."""
+        def mv(vin, vout):
+            v = self.g.unpack_reduced(vin)
+            for _ in range(steps):
+                self.g.single_time_step_iUV_RHS(self.u, v)
+            vout.copy_(self.g.pack_reduced(v))
+            return vout
+        return mv
 
     def tangent(self, steps):
         def mv(vin, vout):

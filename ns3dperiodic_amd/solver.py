@@ -45,8 +45,10 @@ class NS3D:
         flags = ((0 if ipq else _lib.NS3D_FLAG_NO_IPQ) | (_lib.NS3D_FLAG_WENO_ZFIX if weno_zfix else 0)
                  | (_lib.NS3D_FLAG_MASK_NYQUIST if mask_nyquist else 0)
                  | (_lib.NS3D_FLAG_CHECK_FINITE if check_finite else 0))
-        # "fp32": FP32 transforms, FP64 state; "fp32s": also FP32 state inside the RK3 loops
-        prec = {"fp64": _lib.NS3D_FP64, "fp32": _lib.NS3D_FP32_BULK, "fp32s": _lib.NS3D_FP32_STATE}[precision]
+        # "fp32": FP32 transforms, FP64 state; "fp32s": also FP32 state inside the RK3 loops;
+        # "fp32i": FP32 transforms and FP32 stage increments over the FP64 state
+        prec = {"fp64": _lib.NS3D_FP64, "fp32": _lib.NS3D_FP32_BULK, "fp32s": _lib.NS3D_FP32_STATE,
+                "fp32i": _lib.NS3D_FP32_INCR}[precision]
         self.precision = precision
         if stream is None:
             stream = torch.cuda.current_stream(self.device)
@@ -304,6 +306,14 @@ class NS3D:
         out = (ctypes.c_double * 3)()
         check(self.lib.ns3d_probe(self.ctx, *self._c3(u_hat), j, k, l, out), "ns3d_probe")
         return np.array(out[:])
+
+    def probe_batch(self, ub, j, k, l):
+        """probe for every member of a batch ub ([B, 3, Nx, Ny, Mz] complex128): [B, 3] host array."""
+        assert ub.dtype == torch.complex128 and tuple(ub.shape[1:]) == (3,) + self.cshape and ub.is_contiguous()
+        B = int(ub.shape[0])
+        out = (ctypes.c_double * (3 * B))()
+        check(self.lib.ns3d_probe_batch(self.ctx, B, _ptr(ub), j, k, l, out), "ns3d_probe_batch")
+        return np.array(out[:]).reshape(B, 3)
 
     # ------------------------------------------------------------ a11 / a12 (host results)
     def compute_reduction(self, x):

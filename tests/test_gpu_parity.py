@@ -354,7 +354,7 @@ def test_rk3_with_passive_scalar(shape):
     assert rel_l2(to_host([thd])[0], tho) < TOL
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 2e-6), ("fp32s", 2e-6)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-6), ("fp32s", 2e-6), ("fp32i", 2e-6)])
 @pytest.mark.parametrize("shape", [(32, 32, 32), (48, 64, 32)], ids=lambda s: "x".join(map(str, s)))
 def test_rk3_fp32_paths(shape, precision, tol):
     """FP32 paths: five forced RK3 steps in one call against the FP64 oracle

@@ -132,3 +132,74 @@ def test_eigs_close_magnitudes_need_the_restart_buffer():
     out = krylov.eigs(OPS, _mv(A), n, k=6, m=30, which="LM", tol=1e-11)
     for lam in ref:
         assert np.min(np.abs(out["values"] - lam)) < 1e-9, (lam, out["values"])
+
+
+def _rotated_reference(E, J, shift, angle, k, which="LR"):
+    """Dense restatement of the rotated shift-invert eigen mode: the k
+    eigenvalues of the doubled real form M of E - sigma that are smallest in
+    magnitude (largest of M^-1), an orthonormal real basis W of their
+    invariant subspace, R = W_r^T J W_r + W_i^T J W_i, eig(R) sorted."""
+    n = E.shape[0]
+    sr, si = shift * np.cos(angle), shift * np.sin(angle)
+    M = np.block([[E - sr * np.eye(n), si * np.eye(n)], [-si * np.eye(n), E - sr * np.eye(n)]])
+    lam, X = np.linalg.eig(M)
+    idx = np.argsort(np.abs(lam))[:k]
+    Z = np.concatenate([X[:, idx].real, X[:, idx].imag], axis=1)
+    U, s, _ = np.linalg.svd(Z, full_matrices=False)
+    W = U[:, s > 1e-10 * s[0]]
+    Wr, Wi = W[:n], W[n:]
+    R = Wr.T @ J @ Wr + Wi.T @ J @ Wi
+    ev = np.linalg.eigvals(R)
+    return ev[krylov._order(ev, which)][:k]
+
+
+def test_rotated_shift_invert_recovers_eigenvalues_of_j():
+    """With E a function of J (here (I + tau J)^K), E's invariant subspaces are
+    J's, so the Galerkin step returns J's eigenvalues exactly: the ones whose
+    image (1 + tau lambda)^K lies closest to sigma.  Pins the doubled-vector
+    operator (the signs of the rotation), the basis and the recovery."""
+    rng = np.random.default_rng(7)
+    n, tau, K = 160, 0.01, 10
+    Qm, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    D = -np.diag(np.linspace(0.5, 40.0, n))
+    # 0.05 +- 0.45i: E maps it near sigma = 1.01 exp(0.05i) (a complex pair of
+    # J gives one conjugate pair of the doubled operator), real eigenvalues give
+    # two eigenvalues of the doubled operator each (mu - sigma, mu - conj sigma),
+    # so the recovery lists each of them twice, as the reference's does
+    D[:2, :2] = [[0.05, 0.45], [-0.45, 0.05]]
+    D[2, 2], D[3, 3], D[4, 4], D[5, 5] = 0.12, -0.2, -0.6, -0.9
+    J = Qm @ D @ np.linalg.inv(Qm)
+    E = np.linalg.matrix_power(np.eye(n) + tau * J, K)
+    shift, angle = 1.01, 0.05
+    out = krylov.eigs_rotated_shift_invert(OPS, _mv(J), _mv(E), n, k=6, m=30, shift=shift, angle=angle, L=6,
+                                           lin_tol=1e-10, tol=1e-12)
+    ref = _rotated_reference(E, J, shift, angle, 6)
+    lamJ = np.linalg.eigvals(J)
+    for lam in ref:
+        assert np.min(np.abs(lamJ - lam)) < 1e-9            # the reference construction returns J's eigenvalues
+    got = out["values"]
+    assert len(got) == 6
+    for lam in ref:
+        assert np.min(np.abs(got - lam)) < 1e-8 * max(1, abs(lam)), (lam, got)
+    assert np.all(out["residuals"] < 1e-7)
+    for lam in [0.05 + 0.45j, 0.05 - 0.45j, 0.12]:
+        assert np.min(np.abs(got - lam)) < 1e-8
+    assert np.allclose(sorted(out["distinct"], key=lambda z: (z.real, z.imag)),
+                       sorted([0.05 + 0.45j, 0.05 - 0.45j, 0.12, -0.2], key=lambda z: (z.real, z.imag))[-len(out["distinct"]):],
+                       atol=1e-8)
+
+
+def test_rotated_operator_is_the_real_form_of_e_minus_sigma():
+    """The doubled-vector matvec, applied through its inverse, solves
+    (E - sigma) z = b for complex z = x_r + i x_i."""
+    rng = np.random.default_rng(8)
+    n = 40
+    E = rng.standard_normal((n, n)) / np.sqrt(n)
+    shift, angle = 1.3, 0.4
+    sigma = shift * np.exp(1j * angle)
+    inv = krylov.rotated_shift_invert(OPS, _mv(E), n, shift, angle, L=4, tol=1e-13)
+    b = rng.standard_normal(2 * n)
+    x = np.zeros(2 * n)
+    inv(b, x)
+    z = np.linalg.solve(E - sigma * np.eye(n), b[:n] + 1j * b[n:])
+    assert np.allclose(x[:n] + 1j * x[n:], z, atol=1e-10)

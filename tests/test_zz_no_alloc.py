@@ -25,7 +25,7 @@ def _fields(g, N, seed, forced):
     return u0, to_dev(u0), (to_dev(f) if forced else None)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32s"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32s", "fp32i"])
 def test_noalloc_bitwise_equals_rk3_ssp_steps(precision):
     import torch
     from ns3dperiodic_amd import NS3D, _lib
