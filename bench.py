@@ -47,6 +47,20 @@ def step_bytes(nc, bulk_bytes, state_bytes=16):
     return 3 * sum(pb)
 
 
+def pass_bytes_incr(nc):
+    """Algorithmic bytes per launch of each pass, averaged over the three
+    stages, of the FP32-increment RK3 loop (FP32 bulk; the state u^n as the
+    FP32 pair hi + lo; FP32 stage increments d; S = 8 nc bytes):
+    P1 reads hi (3) [+ d (3) in stages 2-3] and writes 6; P2-P4 as in the
+    FP32 path (15, 12, 6); P5 reads Q (3) and hi (3) and writes d (3) in
+    stage 1, also reads d in stage 2 (12), and in stage 3 reads Q, hi, lo, d
+    (12) and writes hi, lo (6) (18).  Per step 3 x (11 + 15 + 12 + 6 + 13) =
+    171 S (the split and join of the caller's FP64 state, once per call, are
+    not counted)."""
+    S = 8 * nc
+    return [11 * S, 15 * S, 12 * S, 6 * S, 13 * S]
+
+
 def profiled_traffic(precision, kernel_prefix):
     """HBM bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)
     of a kernel at 256^3, from the committed profile summary; None if absent."""
@@ -97,7 +111,7 @@ def run_gpu(N, precision, steps, warmup, u0, Re, dt):
     bulk = 16 if precision == "fp64" else 8
     state = 8 if precision == "fp32s" else 16
     nc = N * N * (N // 2 + 1)
-    pb = pass_bytes(nc, bulk, state)
+    pb = pass_bytes_incr(nc) if precision == "fp32i" else pass_bytes(nc, bulk, state)
     passes = []
     for i in range(5):
         avg = ms[i] / calls[i] if calls[i] else float("nan")
@@ -105,7 +119,7 @@ def run_gpu(N, precision, steps, warmup, u0, Re, dt):
         passes.append({"pass": PASS_NAMES[i], "kernel": PASS_KERNELS[i], "calls": calls[i], "avg_ms": avg,
                        "total_ms": ms[i], "alg_bytes": pb[i], "GBps": gbs})
     ms_step = total_ms / steps
-    sb = step_bytes(nc, bulk, state)
+    sb = 3 * sum(pb) if precision == "fp32i" else step_bytes(nc, bulk, state)
     return {
         "precision": precision,
         "steps_per_s": steps / (total_ms * 1e-3),
@@ -147,7 +161,7 @@ def run_scalar(N, steps, warmup, u0, theta0, Re, dt):
             "step_roofline_frac": sb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "finite": finite}
 
 
-def run_accuracy(N, steps, u0, Re, dt, precisions=("fp32", "fp32s")):
+def run_accuracy(N, steps, u0, Re, dt, precisions=("fp32", "fp32s", "fp32i")):
     """Relative L2 distance of each FP32 mode from the FP64 path after `steps`
     RK3 steps from identical initial bits (the FP32 accuracy bar is 1e-5
     after 1000 steps)."""
@@ -196,14 +210,14 @@ def run_tangent(N, precision, steps, u0, Re, dt):
 
 def run_fd_jv_batch(N=64, m=64, Re=150.0, T=1.0, eps=1e-7, seed=1):
     """Batched Krylov work: finite-difference J.v of the time-T map for m
-    seeded unit-norm solenoidal perturbations of an N^3 Kolmogorov-forced
-    field (f_x = sin 4y, Re = 150, k^4 exp(-2(k/4)^2) spectrum), all m + 1 fields
-    advanced together (ns3d_rk3_ssp_batch), against the same m + 1 solves
-    one field at a time.  FP64 (eps = 1e-7 is below FP32 resolution)."""
+    seeded unit-norm solenoidal perturbations of the N^3 Kolmogorov snapshot
+    (f_x = sin 4y, Re = 150, 2000 steps from the seeded k^4 exp(-2(k/4)^2)
+    field), all m + 1 fields advanced together (ns3d_rk3_ssp_batch), against
+    the same m + 1 solves one field at a time.  FP64 (eps = 1e-7 is below
+    FP32 resolution)."""
     import torch
     from ns3dperiodic_amd import NS3D, fields, krylov
-    u0 = fields.random_solenoidal(N, seed=seed, kind="k4exp", k0=4.0, energy=0.5)
-    f = fields.kolmogorov_forcing_sin(N, n=4, amp=1.0)
+    u0, f, t_snap = kolmogorov_snapshot(N, Re, seed=seed)
     umax = max(float(np.abs(np.fft.irfftn(c, s=(N, N, N), axes=(0, 1, 2))).max()) for c in u0)
     steps = int(math.ceil(T / (0.2 * (2 * math.pi / N) / (umax + T))))
     dt = T / steps
@@ -237,10 +251,158 @@ def run_fd_jv_batch(N=64, m=64, Re=150.0, T=1.0, eps=1e-7, seed=1):
     del ws
     g.close()
     return {"workload": f"{m} finite-difference J.v (eps={eps:g}) of the T={T:g} map at {N}^3, Re={Re:g}, "
-                        f"f_x = sin 4y, {steps} RK3 steps of dt={dt:.4g}; {m + 1} fields per batch",
+                        f"f_x = sin 4y, from the snapshot at t = {t_snap:.2f}, {steps} RK3 steps of dt={dt:.4g}; "
+                        f"{m + 1} fields per batch",
             "jv_per_s_batched": m / t_batch, "jv_per_s_one_at_a_time": m / t_seq,
             "speedup": t_seq / t_batch, "batched_s": t_batch, "one_at_a_time_s": t_seq,
             "field_steps_per_s_batched": (m + 1) * steps / t_batch, "bitwise_same_as_alone": bool(same)}
+
+
+def config_dt(u0, N, steps, fmax=1.0, cfl=0.2):
+    """Fixed dt of a forced run: cfl * dx / (max|u0| + steps * dt * fmax) = dt
+    (the velocity bound the forcing allows over the run keeps the CFL number
+    <= cfl), rounded down to two significant digits."""
+    umax = max(float(np.max(np.abs(np.fft.irfftn(c, s=(N, N, N), axes=(0, 1, 2))))) for c in u0)
+    dx = 2 * math.pi / N
+    if fmax == 0.0:
+        dt = cfl * dx / umax
+    else:
+        dt = (-umax + math.sqrt(umax * umax + 4 * steps * fmax * cfl * dx)) / (2 * steps * fmax)
+    e = math.floor(math.log10(dt))
+    return math.floor(dt / 10 ** (e - 1)) * 10 ** (e - 1)
+
+
+def run_config2(N=128, Re=100.0, steps=2000, seed=1, precisions=("fp32", "fp32i")):
+    """Kolmogorov flow (f_x = sin 4y, Re = 100, the seeded k^4 exp(-2(k/4)^2)
+    field of energy 0.5) at N^3 for `steps` RK3 steps in FP64 and in the FP32
+    modes from identical bits; physical energy and dissipation (the spectral
+    sums, every step) compared as series, and the final fields."""
+    import torch
+    from ns3dperiodic_amd import NS3D, fields
+    u0 = fields.random_solenoidal(N, seed=seed, kind="k4exp", k0=4.0, energy=0.5)
+    f = fields.kolmogorov_forcing_sin(N, n=4, amp=1.0)
+    dt = config_dt(u0, N, steps)
+    fd = [torch.from_numpy(c).cuda() for c in f]
+    ser, fin, rate = {}, {}, {}
+    for p in ("fp64",) + tuple(precisions):
+        g = NS3D(N, Re=Re, dt=dt, precision=p, check_finite=True)
+        u = [torch.from_numpy(c).cuda() for c in u0]
+        E, D = np.empty(steps), np.empty(steps)
+        t0 = time.perf_counter()
+        for i in range(steps):
+            g.RK3_SSP(u, fd)
+            E[i] = g.kinetic_energy_spectral(u)
+            D[i] = g.dissipation_spectral(u)
+        rate[p] = steps / (time.perf_counter() - t0)
+        ser[p], fin[p] = (E, D), u
+        g.close()
+    den = sum(float(torch.sum(torch.abs(y) ** 2)) for y in fin["fp64"])
+    e64, d64 = ser["fp64"]
+    out = {"workload": f"Kolmogorov flow {N}^3, f_x = sin 4y, Re = {Re:g}, seeded k^4 exp(-2(k/4)^2) field (energy 0.5), "
+                       f"{steps} RK3 steps of dt = {dt:g} (T = {steps * dt:.3f}); energy and dissipation every step",
+           "dt": dt, "energy_start_end": [float(e64[0]), float(e64[-1])],
+           "dissipation_start_end": [float(d64[0]), float(d64[-1])],
+           "steps_per_s_with_diagnostics": {names_of(p): r for p, r in rate.items()}}
+    for p in precisions:
+        e, d = ser[p]
+        out[names_of(p)] = {
+            "max_rel_diff_energy": float(np.max(np.abs(e - e64) / np.abs(e64))),
+            "max_rel_diff_dissipation": float(np.max(np.abs(d - d64) / np.abs(d64))),
+            "final_field_rel_l2": math.sqrt(sum(float(torch.sum(torch.abs(x - y) ** 2))
+                                                for x, y in zip(fin[p], fin["fp64"])) / den)}
+    del fin
+    torch.cuda.empty_cache()
+    return out
+
+
+def kolmogorov_snapshot(N=64, Re=150.0, steps=2000, seed=1):
+    """The state after `steps` RK3 steps of the Kolmogorov-flow recipe above
+    at N^3 and Re (FP64): the base state of the batched Krylov work."""
+    import torch
+    from ns3dperiodic_amd import NS3D, fields
+    u0 = fields.random_solenoidal(N, seed=seed, kind="k4exp", k0=4.0, energy=0.5)
+    f = fields.kolmogorov_forcing_sin(N, n=4, amp=1.0)
+    dt = config_dt(u0, N, steps)
+    g = NS3D(N, Re=Re, dt=dt)
+    u = [torch.from_numpy(c).cuda() for c in u0]
+    g.RK3_SSP(u, [torch.from_numpy(c).cuda() for c in f], steps=steps)
+    g.close()
+    return [c.cpu().numpy() for c in u], f, dt * steps
+
+
+def run_config4_krylov(N=64, Re=150.0, T=1.0, eps=1e-7, k=6, m=30, seed=1):
+    """The Krylov legs on the 64^3 Re = 150 Kolmogorov snapshot:
+    * Arnoldi (IRA, m = 30, 6 wanted, "LM", FP64 Gram-Schmidt with DGKS) on
+      the finite-difference J.v of the T = 1 map, v -> (Phi(u + eps v) -
+      Phi(u)) / eps with Phi(u) computed once: the leading Floquet multipliers;
+    * Newton-Krylov for the steady Kolmogorov state from the laminar profile
+      plus a 1e-3 perturbation: FP32 Jacobian (J.v from an FP32-bulk context)
+      inside BiCGStab(2), residuals F(x) in FP64."""
+    import torch
+    from ns3dperiodic_amd import NS3D, fields, krylov
+    us, f, t_snap = kolmogorov_snapshot(N, Re, seed=seed)
+    umax = max(float(np.abs(np.fft.irfftn(c, s=(N, N, N), axes=(0, 1, 2))).max()) for c in us)
+    steps = int(math.ceil(T / (0.2 * (2 * math.pi / N) / (umax + T))))
+    dt = T / steps
+    g = NS3D(N, Re=Re, dt=dt)
+    fd = [torch.from_numpy(c).cuda() for c in f]
+    u = [torch.from_numpy(c).cuda() for c in us]
+    base = [c.clone() for c in u]
+    g.RK3_SSP(base, fd, steps=steps)
+    n = g.vector_length_reduced()
+    ops = krylov.GpuOps(g)
+    nmv = [0]
+
+    def fdjv(vin, vout):
+        v = g.unpack_reduced(vin)
+        w = [(a + eps * b).contiguous() for a, b in zip(u, v)]
+        g.RK3_SSP(w, fd, steps=steps)
+        vout.copy_(g.pack_reduced([(a - b) / eps for a, b in zip(w, base)]))
+        nmv[0] += 1
+        return vout
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = krylov.eigs(ops, fdjv, n, k=k, m=m, which="LM", tol=1e-6, max_iter=60, residuals=False)
+    t_arn = time.perf_counter() - t0
+    g.close()
+    # Newton-Krylov: F(x) = L(x) (return_RHS) in FP64, J from an FP32-bulk context
+    ust = [(Re / 16.0) * c for c in f]          # laminar state of f_x = sin 4y: -|k|^2/Re u + f = 0
+    g64 = NS3D(N, Re=Re, dt=dt)
+    g32 = NS3D(N, Re=Re, dt=dt, precision="fp32")
+    op64 = krylov.NSOperators(g64, None, fd)
+    op32 = krylov.NSOperators(g32, None, fd, tau=5.0)      # the reference's globalisation, tau = 5
+    rng = np.random.default_rng(seed)
+    x = g64.pack_reduced([torch.from_numpy(c).cuda() for c in ust])
+    x += 1e-3 * torch.from_numpy(rng.standard_normal(n)).cuda() * float(torch.linalg.vector_norm(x)) / math.sqrt(n)
+    F0 = op64.rhs(x, torch.empty_like(x))
+    f0 = float(torch.linalg.vector_norm(F0))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nt = krylov.newton(op64.ops, op64.rhs, op32.jacobian, x, tol=1e-8 * f0, max_iter=20, L=2, lin_tol=1e-6,
+                       lin_iter=200)
+    t_newton = time.perf_counter() - t0
+    g64.close()
+    g32.close()
+    return {"workload": f"{N}^3 Kolmogorov snapshot (Re = {Re:g}, f_x = sin 4y, t = {t_snap:.2f} from the seeded field); "
+                        f"J.v of the T = {T:g} map by forward differences (eps = {eps:g}, {steps} RK3 steps of "
+                        f"dt = {dt:.4g})",
+            "arnoldi": {"k": k, "m": m, "which": "LM", "restarts": r["iterations"], "ritz_norm": r["ritz_norm"],
+                        "matvecs": nmv[0], "seconds": t_arn, "matvecs_per_s": nmv[0] / t_arn,
+                        "multipliers_abs": [float(abs(z)) for z in r["values"]]},
+            "newton": {"start": "laminar profile (Re/16) sin 4y + 1e-3 relative random perturbation",
+                       "jacobian": "FP32 bulk J.v - v/tau (tau = 5, NSCallMatrixVector_reduced_Newton_Globalization), "
+                                   "BiCGStab(2) to 1e-6, at most 200 iterations", "residual": "FP64 F(x) = L(x)",
+                       "tol": "||F|| < 1e-8 ||F(x0)||", "breakdown_flag": nt.get("breakdown"),
+                       "converged": nt["converged"], "iterations": nt["iterations"],
+                       "residual_history": [float(h) for h in nt["history"]],
+                       "bicgstab_iterations": [int(a) for a, _ in nt["lin_history"]], "seconds": t_newton}}
+
+
+def run_accuracy_512(u512, Re, dt, steps, precisions=("fp32", "fp32s", "fp32i")):
+    """Config 3 at 512^3: relative L2 distance of each FP32 mode from FP64
+    after `steps` steps from identical bits (one precision resident at a time)."""
+    return run_accuracy(512, steps, u512, Re, dt, precisions)
 
 
 def pad_spectrum(u, M):
@@ -485,7 +647,7 @@ def run_serving(n_req, rate, seed, precision, max_active, chunk, rm_fraction=0.1
 
 
 def names_of(precision):
-    return {"fp64": "fp64", "fp32": "fp32_bulk", "fp32s": "fp32_state"}[precision]
+    return {"fp64": "fp64", "fp32": "fp32_bulk", "fp32s": "fp32_state", "fp32i": "fp32_incr"}[precision]
 
 
 def main():
@@ -494,9 +656,10 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--N", type=int, default=256)
-    ap.add_argument("--precision", choices=["fp64", "fp32", "fp32s"], default="fp64")
-    ap.add_argument("--secondary", default="fp32,fp32s",
-                    help="comma list of further precisions to time beside the primary (fp64, fp32, fp32s) or none")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "fp32s", "fp32i"], default="fp64")
+    ap.add_argument("--secondary", default="fp32,fp32s,fp32i",
+                    help="comma list of further precisions to time beside the primary (fp64, fp32, fp32s, fp32i) "
+                         "or none")
     ap.add_argument("--cpu-steps", type=int, default=10,
                     help="oracle steps timed one by one after a warm-up step (the CPU baseline is their median)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -511,6 +674,8 @@ def main():
                     help="run only a plain 1 GiB device copy (3 times): the reference kernel for HBM counters")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="RK3 steps after which the FP32 modes are compared with FP64 (0: skip)")
+    ap.add_argument("--accuracy-steps-512", type=int, default=1000,
+                    help="steps of the 512^3 FP32-vs-FP64 accuracy leg (0: skip)")
     ap.add_argument("--no-extra", action="store_true",
                     help="only the main timing: skip the FP32 accuracy, 512^3, passive-scalar, tangent-map, "
                          "batched J.v and copy-peak legs")
@@ -566,6 +731,13 @@ def main():
                 (x for x in secs if x["precision"] == t["precision"]), None)
             t["vs_plain_step"] = t["ms_per_step"] / plain["ms_per_step"] if plain else None
             print(f"[bench] tangent map {t['precision']}: {t['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+        extra["config2"] = run_config2(seed=args.seed)
+        print(f"[bench] config 2 (128^3 Kolmogorov, 2000 steps): "
+              f"{ {k: v for k, v in extra['config2'].items() if k.startswith('fp32')} }", file=sys.stderr, flush=True)
+        extra["config4"] = run_config4_krylov(seed=args.seed)
+        print(f"[bench] config 4 Krylov: arnoldi {extra['config4']['arnoldi']['matvecs']} matvecs in "
+              f"{extra['config4']['arnoldi']['seconds']:.1f} s, newton {extra['config4']['newton']['iterations']} "
+              f"iterations", file=sys.stderr, flush=True)
         extra["fd_jv_batch"] = run_fd_jv_batch(seed=args.seed)
         print(f"[bench] batched FD J.v: {extra['fd_jv_batch']['jv_per_s_batched']:.1f} J.v/s "
               f"({extra['fd_jv_batch']['speedup']:.2f}x one at a time)", file=sys.stderr, flush=True)
@@ -578,13 +750,17 @@ def main():
             u512 = pad_spectrum(u0, M)
             dt512 = float("%.1e" % (dt * N / M))
             big = {"workload": f"the {N}^3 field zero-padded to {M}^3, same spectrum, dt = {dt512:g}", "dt": dt512}
-            for p in ["fp64", "fp32", "fp32s"]:
+            for p in ["fp64", "fp32", "fp32s", "fp32i"]:
                 r = run_gpu(M, p, 5, 1, u512, Re, dt512)
                 big[names_of(p)] = {"steps_per_s": r["steps_per_s"], "ms_per_step": r["ms_per_step"],
                                     "step_alg_bytes": r["step_alg_bytes"],
                                     "step_roofline_frac": r["step_GBps"] / HBM_PEAK_GBS, "finite": r["finite"],
                                     "passes_avg_ms": {q["pass"]: q["avg_ms"] for q in r["passes"]}}
                 print(f"[bench] {M}^3 {p}: {r['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+            if args.accuracy_steps_512 > 0:
+                big["fp32_accuracy"] = run_accuracy_512(u512, Re, dt512, args.accuracy_steps_512)
+                print(f"[bench] {M}^3 rel L2 vs fp64 after {args.accuracy_steps_512} steps: "
+                      f"{big['fp32_accuracy']}", file=sys.stderr, flush=True)
             del u512
             extra["n512"] = big
 
@@ -622,7 +798,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32 bulk (f64 state)", "fp32s": "f32 (f32 state)"}[args.precision],
+        "dtype": {"fp64": "f64", "fp32": "f32 bulk (f64 state)", "fp32s": "f32 (f32 state)",
+                  "fp32i": "f32 bulk and stage increments (f64 state)"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"decaying turbulence {N}^3, nu=2e-3, k0=4 spectrum, reference mode",
                    "N": N, "Re": Re, "dt": dt, "global_batch": 1, "parallelism": "single GPU"},
@@ -642,6 +819,8 @@ def main():
                                 dominant_frac_of_copy=dom["GBps"] / cp)
         out["scalar"] = extra["scalar"]
         out["fd_jv_batch"] = extra["fd_jv_batch"]
+        out["config2"] = extra["config2"]
+        out["config4_krylov"] = extra["config4"]
         out["step_graph"] = extra["step_graph"]
         out["tangent_map"] = {"workload": f"{N}^3 RK3_SSP_UV steps (base + one perturbation)",
                               "legs": extra["tangent"]}

@@ -44,9 +44,10 @@ typedef struct { double x, y; } ns3d_cplx;
 #define NS3D_FP32_BULK 1   /* FP32 transforms and products; state and epilogue FP64 */
 #define NS3D_FP32_STATE 2  /* as FP32_BULK, and the RK3 loops (ns3d_rk3_ssp*, graph) keep the state and
                               stages in FP32, converting the caller's FP64 state once per call */
-#define NS3D_FP32_INCR 3   /* as FP32_BULK, and the RK3 loops carry the SSP stages as FP32 increments over
-                              the FP64 state u^n (u1 = u^n + d1, u2 = u^n + d2) and feed the transforms an
-                              FP32 copy of u^n: FP32 stage storage, FP64 state accuracy */
+#define NS3D_FP32_INCR 3   /* as FP32_BULK, and the RK3 loops hold the state u^n as a pair of FP32 fields
+                              hi + lo (converted from and to the caller's FP64 state once per call) and carry
+                              the SSP stages as FP32 increments over it (u1 = u^n + d1, u2 = u^n + d2); the
+                              transforms read hi: FP32 storage throughout, near-FP64 state accuracy */
 
 /* behaviour flags (reference quirks are ON in reference mode) */
 #define NS3D_FLAG_NO_IPQ 0x1u      /* disable zeroing Re(Q^) and Q^(0) (advection_2_3.cu:35-70) */

@@ -44,22 +44,27 @@ SPACE = {
     "NS3D_PZT64": (128, [64, 128, 256], ["fp64"], "pow2"),
     "NS3D_ZSWZ": (16, [0, 16], ["fp64"], "pow2"),
     "NS3D_ZILP": (1, [0, 1], ["fp64"], "pow2"),
-    "NS3D_XFWD_ILP": (2, [0, 1, 2], ["fp64", "fp32", "fp32s"], "pow2"),
-    "NS3D_EXINV32": (8, [4, 8], ["fp32", "fp32s"], "pow2"),
-    "NS3D_EYINV32": (4, [4, 8], ["fp32", "fp32s"], "pow2"),
-    "NS3D_PXINV32": (8, [4, 8, 16], ["fp32", "fp32s"], "pow2"),
-    "NS3D_PYINV32": (8, [4, 8, 16], ["fp32", "fp32s"], "pow2"),
-    "NS3D_PYFWD32": (16, [8, 16], ["fp32", "fp32s"], "pow2"),
-    "NS3D_PXFWD32": (8, [4, 8, 16], ["fp32", "fp32s"], "pow2"),
-    "NS3D_PZT32": (128, [64, 128, 256], ["fp32", "fp32s"], "pow2"),
-    "NS3D_ZPLAN32": (8, [4, 8], ["fp32", "fp32s"], "pow2"),
-    "NS3D_ZSWZ32": (16, [0, 16], ["fp32", "fp32s"], "pow2"),
+    "NS3D_XFWD_ILP": (2, [0, 1, 2], ["fp64", "fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_EXINV32": (8, [4, 8], ["fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_EYINV32": (4, [4, 8], ["fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_PXINV32": (8, [4, 8, 16], ["fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_PYINV32": (8, [4, 8, 16], ["fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_PYFWD32": (16, [8, 16], ["fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_PXFWD32": (8, [4, 8, 16], ["fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_PZT32": (128, [64, 128, 256], ["fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_ZPLAN32": (8, [4, 8], ["fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_ZSWZ32": (16, [0, 16], ["fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_EXFWD32": (4, [4, 8], ["fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_ZILP32": (0, [0, 1], ["fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_ZWAVE": (1, [0, 1], ["fp64", "fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_WAVES_Z32": (2, [2, 3], ["fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_WAVES_XFWD32": (2, [2, 3, 4], ["fp32", "fp32s", "fp32i"], "pow2"),
     "NS3D_R3_PXINV64": (4, [4, 8], ["fp64"], "r3"),
     "NS3D_R3_PYINV64": (4, [4, 8], ["fp64"], "r3"),
     "NS3D_R3_PZT64": (256, [128, 256], ["fp64"], "r3"),
     "NS3D_R3_ZSWZ": (0, [0, 16], ["fp64"], "r3"),
-    "NS3D_R3_PXFWD32": (16, [8, 16], ["fp32", "fp32s"], "r3"),
-    "NS3D_R3_PYINV32": (4, [4, 8], ["fp32", "fp32s"], "r3"),
+    "NS3D_R3_PXFWD32": (16, [8, 16], ["fp32", "fp32s", "fp32i"], "r3"),
+    "NS3D_R3_PYINV32": (4, [4, 8], ["fp32", "fp32s", "fp32i"], "r3"),
 }
 
 
@@ -80,11 +85,14 @@ def baseline():
     return cur
 
 
-def variants():
-    """(name, overrides) for the baseline and every single-parameter change."""
+def variants(only=None):
+    """(name, overrides) for the baseline and every single-parameter change
+    (of the parameters in `only`, if given)."""
     base = baseline()
     out = [("base", {})]
     for k, (_, cands, _, _) in SPACE.items():
+        if only and k not in only:
+            continue
         for c in cands:
             if c != base[k]:
                 out.append((f"{k}={c}", {k: c}))
@@ -108,7 +116,7 @@ def _build_one(out_dir, name, overrides, base):
 
 def cmd_build(args):
     os.makedirs(args.out, exist_ok=True)
-    base, vs = variants()
+    base, vs = variants(args.only.split(",") if args.only else None)
     print(f"[autotune] baseline {base}", flush=True)
     print(f"[autotune] building {len(vs)} variants with {args.jobs} jobs", flush=True)
     plan = []
@@ -158,8 +166,9 @@ def cmd_time(args):
             if v["overrides"]:
                 _, _, precs, cls = SPACE[next(iter(v["overrides"]))]
                 runs = [(cls, precs)]
-            else:   # the baseline at both sizes
-                runs = [(cls, ["fp64", "fp32", "fp32s"]) for cls in sizes]
+            else:   # the baseline at both sizes (only the classes the plan's variants need)
+                need = {SPACE[next(iter(x["overrides"]))][3] for x in plan["variants"] if x["overrides"]}
+                runs = [(cls, ["fp64", "fp32", "fp32s", "fp32i"]) for cls in sizes if cls in need or not need]
             for cls, precs in runs:
                 N, u0, dt = work[cls]
                 for p in precs:
@@ -181,8 +190,12 @@ def cmd_select(args):
     ms = t["ms_per_step"]
     base = plan["baseline"]
     chosen, notes = dict(base), []
+    planned = {next(iter(v["overrides"])) for v in plan["variants"] if v["overrides"]}
     for k, (_, cands, precs, cls) in SPACE.items():
         N = t["N"] if cls == "pow2" else t["N3"]
+        if k not in planned:        # not timed in this run: the current value stays
+            notes.append(f"{k} = {base[k]} (not timed in the last run)")
+            continue
 
         def cost(name):
             return sum(ms[f"{name}@{N}|{p}"] for p in precs)
@@ -219,6 +232,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--margin", type=float, default=0.005)
+    ap.add_argument("--only", default="", help="comma list of parameters to vary (build); default all")
     args = ap.parse_args()
     {"build": cmd_build, "time": cmd_time, "select": cmd_select}[args.phase](args)
 

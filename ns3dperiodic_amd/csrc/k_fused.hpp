@@ -35,6 +35,14 @@ template <typename T, int N> struct PtrArr { T* p[N]; };
 #ifndef NS3D_WAVES_Z
 #define NS3D_WAVES_Z 2
 #endif
+// Minimum waves per SIMD of the FP32 z pass and of the FP32 x-forward
+// epilogue pass (their register budgets: 2 -> 256 VGPRs, 3 -> 168, 4 -> 128).
+#ifndef NS3D_WAVES_Z32
+#define NS3D_WAVES_Z32 2
+#endif
+#ifndef NS3D_WAVES_XFWD32
+#define NS3D_WAVES_XFWD32 2
+#endif
 // x/y-pass plan: E = 8 points per thread (Plan) or 4 (ZPlan); chosen per kernel
 // and precision by the host from measurements.
 template <int N, int E> struct XYSel { using type = Plan<N>; };
@@ -338,6 +346,9 @@ struct ZPencil {
                 const int n = FI::in_index(t, q, r);
                 const int sl = n <= NZ / 2 ? n : NZ - n;
                 cplx<TB> a = mk<TB>(0, 0), b = mk<TB>(0, 0);
+                // masked and truncated elements stay zero without a load (a branch-free
+                // variant that loaded a clamped in-bounds element and zeroed it instead
+                // made the 256^3 z pass 3.4x slower in FP64, 1.6x in FP32)
                 if (FI::in_active(t, q) && valid && (!TRUNC || sl <= tab.lmax)) {
                     a = s1[sl];
                     b = s2[sl];
@@ -457,7 +468,8 @@ template <typename TB, int NZ, int P, int NF>
 #ifndef NS3D_WAVES_Z2
 #define NS3D_WAVES_Z2 1
 #endif
-__global__ void __attribute__((amdgpu_waves_per_eu(NF == 1 ? NS3D_WAVES_Z : NS3D_WAVES_Z2, 8))) __launch_bounds__((P * ZPlanT<TB, NZ>::TPP))
+__global__ void __attribute__((amdgpu_waves_per_eu(NF == 1 ? (sizeof(TB) == 4 ? NS3D_WAVES_Z32 : NS3D_WAVES_Z) : NS3D_WAVES_Z2, 8)))
+__launch_bounds__((P * ZPlanT<TB, NZ>::TPP))
 k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, long npen, int Mz, int nb,
             SpecTables tab, const cplx<TB>* __restrict__ twg, TB scale, TB* __restrict__ uphys, long nr, long bsc = 0) {
     // B is zero on z-planes l > lmax (never written by the y pass): TRUNC skips them
@@ -685,15 +697,17 @@ k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, i
 // ---------------------------------------------------------------- P5 epilogues
 // EPI_STAGE32: EPI_STAGE with w, u0 and out in FP32 (the FP32-state RK3 loop);
 // the arithmetic and the forcing stay FP64.
-// EPI_INC / EPI_INC_FINAL: the SSP stages in increment form over an FP64 base
-// state (the FP32-increment RK3 loop).  With w = base + d (d: the previous
-// stage's FP32 increment, u0; null in the first stage):
-//   EPI_INC        out   = a1*(d + dt*L(w))          base FP32 (w), out FP32
-//   EPI_INC_FINAL  out   = base + a1*(d + dt*L(w))   base FP64 (w), out FP64,
-//                  out32 = the FP32 copy of out
+// EPI_INC / EPI_INC_FINAL: the SSP stages in increment form over a base state
+// u^n held as a pair of FP32 fields, u^n = hi + lo (the FP32-increment RK3
+// loop).  With w = base + d (d: the previous stage's FP32 increment, u0; null
+// in the first stage):
+//   EPI_INC        out = a1*(d + dt*L(w)),  base = hi (w)            out FP32
+//   EPI_INC_FINAL  r = base + a1*(d + dt*L(w)),  base = hi + lo (w, out32):
+//                  hi = fp32(r) written to w, lo = fp32(r - hi) to out32
 // which is RK3_SSP's u1 = u + dt L(u), u2 = 3/4 u + 1/4 (u1 + dt L(u1)),
 // u = 1/3 u + 2/3 (u2 + dt L(u2)) with a1 = 1, 1/4, 2/3 and the stage values
-// u1 = u + d1, u2 = u + d2.
+// u1 = u + d1, u2 = u + d2.  out may alias u0 (each mode is read, then written,
+// by one thread).
 enum EpiMode {
     EPI_Q = 0, EPI_STAGE = 1, EPI_RHS = 2, EPI_UV = 3, EPI_UV_RHS = 4, EPI_IUV = 5, EPI_STAGE32 = 6,
     EPI_INC = 7, EPI_INC_FINAL = 8
@@ -705,6 +719,7 @@ struct EpiArgs {
     const cplx<double>* f[3];    // forcing (EPI_STAGE / EPI_RHS, may be null)
     cplx<double>* out[3];        // output (EPI_Q, EPI_STAGE, EPI_RHS); cplx<float> for EPI_STAGE32
     double dt, Re, a0, a1;       // EPI_STAGE: out = a0*u0 + a1*E(w)
+    double nu, dtnu;             // 1/Re and dt/Re (formed once on the host)
     int ipq;                     // reference IPQ rule (Q1)
     // mode (j,k,l) of w / u0 / out / f sits at (j*ny + k)*m + l: m = Mz for the
     // reference layout, a padded row stride for the library's own stage buffers
@@ -713,9 +728,8 @@ struct EpiArgs {
     long bw, bu0, bo;
     // sticky non-finite flag: set to 1 by any lane that writes a NaN or Inf (may be null)
     int* nf;
-    // EPI_INC_FINAL: FP32 copy of the new state, rows of stride m32
+    // EPI_INC_FINAL: the lo half of the state (read and written), rows of stride mw
     cplx<float>* out32[3];
-    int m32;
 };
 
 // Non-finite flag of the fused epilogues and the scalar stage: every lane that
@@ -746,13 +760,13 @@ __device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& ta
     const double AQi[3] = {c.a11 * Qx.y + c.a12 * Qy.y + c.a13 * Qz.y, c.a12 * Qx.y + c.a22 * Qy.y + c.a23 * Qz.y,
                            c.a13 * Qx.y + c.a23 * Qy.y + c.a33 * Qz.y};
     const cplx<double> Q[3] = {Qx, Qy, Qz};
-    const double dt = ea.dt, Re = ea.Re;
+    const double dt = ea.dt;
 #pragma unroll
     for (int b = 0; b < 3; b++) {
         cplx<double> r;
         if (MODE == EPI_STAGE32) {
             const cplx<double> w = cvt<double>(reinterpret_cast<const cplx<float>*>(ea.w[b])[gw]);
-            const double G = (1.0 - c.din * dt / Re);
+            const double G = (1.0 - c.din * ea.dtnu);
             cplx<double> f = mk<double>(0.0, 0.0);
             if (ea.f[b]) f = ea.f[b][gf];
             r.x = dt * AQr[b] + G * w.x + dt * f.x - dt * Q[b].x;
@@ -768,13 +782,18 @@ __device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& ta
             continue;
         }
         if (MODE == EPI_INC || MODE == EPI_INC_FINAL) {
-            const cplx<double> base =
-                MODE == EPI_INC_FINAL ? ea.w[b][gw] : cvt<double>(reinterpret_cast<const cplx<float>*>(ea.w[b])[gw]);
+            cplx<float>* hi = reinterpret_cast<cplx<float>*>(ea.w[b]);
+            cplx<double> base = cvt<double>(hi[gw]);
+            if (MODE == EPI_INC_FINAL) {
+                const cplx<float> lo = ea.out32[b][gw];
+                base.x += (double)lo.x;
+                base.y += (double)lo.y;
+            }
             const cplx<double> d =
                 ea.u0[b] ? cvt<double>(reinterpret_cast<const cplx<float>*>(ea.u0[b])[gu]) : mk<double>(0.0, 0.0);
             cplx<double> f = mk<double>(0.0, 0.0);
             if (ea.f[b]) f = ea.f[b][gf];
-            const double Gd = c.din * dt / Re;
+            const double Gd = c.din * ea.dtnu;
             r.x = ea.a1 * (d.x + (dt * AQr[b] - Gd * (base.x + d.x) + dt * f.x - dt * Q[b].x));
             r.y = ea.a1 * (d.y + (dt * AQi[b] - Gd * (base.y + d.y) + dt * f.y - dt * Q[b].y));
             if (MODE == EPI_INC) {
@@ -784,15 +803,16 @@ __device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& ta
             } else {
                 r.x += base.x;
                 r.y += base.y;
-                ea.out[b][go] = r;
-                ea.out32[b][row * ea.m32 + l] = cvt<float>(r);
+                const cplx<float> h = cvt<float>(r);
+                hi[gw] = h;
+                ea.out32[b][gw] = mk<float>((float)(r.x - (double)h.x), (float)(r.y - (double)h.y));
                 bad |= !finite2(r);
             }
             continue;
         }
         const cplx<double> w = ea.w[b][gw];
         if (MODE == EPI_STAGE) {
-            const double G = (1.0 - c.din * dt / Re);
+            const double G = (1.0 - c.din * ea.dtnu);
             cplx<double> f = mk<double>(0.0, 0.0);
             if (ea.f[b]) f = ea.f[b][gf];
             r.x = dt * AQr[b] + G * w.x + dt * f.x - dt * Q[b].x;
@@ -805,7 +825,7 @@ __device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& ta
             ea.out[b][go] = r;
             bad |= !finite2(r);
         } else if (MODE == EPI_RHS) {
-            const double G = (-c.din / Re);
+            const double G = (-c.din * ea.nu);
             cplx<double> f = mk<double>(0.0, 0.0);
             if (ea.f[b]) f = ea.f[b][gf];
             r.x = AQr[b] + G * w.x + f.x - Q[b].x;
@@ -813,19 +833,19 @@ __device__ __forceinline__ void epilogue(const EpiArgs& ea, const SpecTables& ta
             ea.out[b][go] = r;
             bad |= !finite2(r);
         } else if (MODE == EPI_UV) {
-            const double G = (1.0 - c.din * dt / Re);
+            const double G = (1.0 - c.din * ea.dtnu);
             r.x = dt * AQr[b] + G * w.x - dt * Q[b].x;
             r.y = dt * AQi[b] + G * w.y - dt * Q[b].y;
             ea.w[b][gw] = r;
             bad |= !finite2(r);
         } else if (MODE == EPI_UV_RHS) {
-            const double G = (-c.din / Re);
+            const double G = (-c.din * ea.nu);
             r.x = AQr[b] + G * w.x - Q[b].x;
             r.y = AQi[b] + G * w.y - Q[b].y;
             ea.w[b][gw] = r;
             bad |= !finite2(r);
         } else {  // EPI_IUV
-            const double G = (1.0 + c.din * dt / Re);
+            const double G = (1.0 + c.din * ea.dtnu);
             r.x = (dt * AQr[b] + w.x - dt * Q[b].x) / G;
             r.y = (dt * AQi[b] + w.y - dt * Q[b].y) / G;
             ea.w[b][gw] = r;
@@ -850,7 +870,7 @@ __device__ __forceinline__ void ipq_rule(cplx<double>& Qx, cplx<double>& Qy, cpl
 }
 
 template <typename TB, int NX, int P, int MODE, int E>
-__global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
+__global__ void __attribute__((amdgpu_waves_per_eu(sizeof(TB) == 4 ? NS3D_WAVES_XFWD32 : NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
 k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int nlt, int nb, SpecTables tab,
             const cplx<TB>* __restrict__ twg, EpiArgs ea, long bsc = 0) {
     constexpr int TPP = XYPlan<NX, E>::TPP;

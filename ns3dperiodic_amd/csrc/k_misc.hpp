@@ -94,6 +94,27 @@ __global__ void k_f2d_rows(const cplx<float>* __restrict__ in, cplx<double>* __r
     out[i] = cvt<double>(in[row * Mp + (i - row * Mz)]);
 }
 
+// FP64 field in the reference layout <-> the pair (hi, lo) of FP32 fields with
+// rows padded to Mp: hi = fp32(u), lo = fp32(u - hi) (u = hi + lo to ~2^-48)
+__global__ void k_split_rows(const cplx<double>* __restrict__ in, cplx<float>* __restrict__ hi, cplx<float>* __restrict__ lo,
+                             long n, int Mz, int Mp) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long row = i / Mz, o = row * Mp + (i - row * Mz);
+    const cplx<double> u = in[i];
+    const cplx<float> h = cvt<float>(u);
+    hi[o] = h;
+    lo[o] = mk<float>((float)(u.x - (double)h.x), (float)(u.y - (double)h.y));
+}
+__global__ void k_join_rows(const cplx<float>* __restrict__ hi, const cplx<float>* __restrict__ lo,
+                            cplx<double>* __restrict__ out, long n, int Mz, int Mp) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long row = i / Mz, o = row * Mp + (i - row * Mz);
+    const cplx<float> h = hi[o], l = lo[o];
+    out[i] = mk<double>((double)h.x + (double)l.x, (double)h.y + (double)l.y);
+}
+
 // ------------------------------------------------------- a7: copy / axpby on 3 fields
 // single_RK_step_device (Source/RK_time_step.cu:82-115): dst = w0*s0 + w1*s1.
 __global__ void __launch_bounds__(256)

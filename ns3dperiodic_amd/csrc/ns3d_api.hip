@@ -105,7 +105,10 @@ constexpr int PXY_T = (sizeof(TB) == 8 ? P64 : P32) * XYPlan<N, E>::TPP <= NS3D_
 template <typename TB> constexpr int E_XINV = sizeof(TB) == 8 ? NS3D_EXINV64 : NS3D_EXINV32;
 template <typename TB> constexpr int E_YINV = sizeof(TB) == 8 ? NS3D_EYINV64 : NS3D_EYINV32;
 template <typename TB> constexpr int E_C2C = 8;
-template <typename TB> constexpr int E_XFWD = sizeof(TB) == 8 ? NS3D_EXFWD64 : 4;
+#ifndef NS3D_EXFWD32
+#define NS3D_EXFWD32 4
+#endif
+template <typename TB> constexpr int E_XFWD = sizeof(TB) == 8 ? NS3D_EXFWD64 : NS3D_EXFWD32;
 // one output field per block in the inverse passes (input re-read by 2 or 3 blocks)
 #ifndef NS3D_OS_XINV
 #define NS3D_OS_XINV 0
@@ -559,6 +562,8 @@ EpiArgs make_epi(const ns3d_ctx* c) {
     memset(&ea, 0, sizeof(ea));
     ea.dt = c->dt;
     ea.Re = c->Re;
+    ea.nu = 1.0 / c->Re;
+    ea.dtnu = c->dt / c->Re;
     ea.a0 = 0.0;
     ea.a1 = 1.0;
     ea.ipq = (c->flags & NS3D_FLAG_NO_IPQ) ? 0 : 1;
@@ -672,46 +677,43 @@ int ensure_state32(ns3d_ctx* c) {
     return NS3D_OK;
 }
 
-// ---- NS3D_FP32_INCR: FP32 transforms and products over an FP64 state.  The
-// RK3 stages are carried as FP32 increments d1, d2 over the FP64 state u^n
-// (u1 = u^n + d1, u2 = u^n + d2; EPI_INC / EPI_INC_FINAL), so the rounding of
-// a stage value is eps32*|d| instead of eps32*|u|; the transforms read an FP32
-// copy of u^n (s32[0..2], rows m16) that the last stage rewrites with u^{n+1}.
-// s32: copy of u^n [3], d1 [3], d2 [3].
-int stage_incr(ns3d_ctx* c, const cf* const d[3], double a1, cf* const out[3], const zd* const f[3],
-               zd* const u_final[3]) {
+// ---- NS3D_FP32_INCR: FP32 transforms and products over an FP64-accurate
+// state.  Inside the RK3 loop the state u^n is the pair hi + lo of FP32
+// fields (converted from and to the caller's FP64 field once per call) and
+// the SSP stages are FP32 increments d over it (u1 = u^n + d1, u2 = u^n + d2;
+// EPI_INC / EPI_INC_FINAL), so a stage value is rounded to eps32*|d|, not
+// eps32*|u|.  The transforms and the stages' diffusion read hi alone; the
+// last stage reads hi + lo and writes the new pair.  d2 overwrites d1 in place.
+// s32 (rows m16): hi [3], lo [3], d [3].
+int stage_incr(ns3d_ctx* c, bool first, double a1, bool last, const zd* const f[3]) {
     const long nc = c->nc16;
-    const cf* ub[3] = {c->s32, c->s32 + nc, c->s32 + 2 * nc};
+    cf* hi[3] = {c->s32, c->s32 + nc, c->s32 + 2 * nc};
+    cf* lo[3] = {c->s32 + 3 * nc, c->s32 + 4 * nc, c->s32 + 5 * nc};
+    cf* d[3] = {c->s32 + 6 * nc, c->s32 + 7 * nc, c->s32 + 8 * nc};
+    const cf* hic[3] = {hi[0], hi[1], hi[2]};
+    const cf* dc[3] = {d[0], d[1], d[2]};
     EpiArgs ea = make_epi(c);
     for (int b = 0; b < 3; b++) {
-        ea.w[b] = u_final ? u_final[b] : (zd*)ub[b];
-        ea.u0[b] = d ? (const zd*)d[b] : nullptr;
+        ea.w[b] = (zd*)hi[b];
+        ea.u0[b] = first ? nullptr : (const zd*)d[b];
         ea.f[b] = f ? f[b] : nullptr;
-        ea.out[b] = u_final ? u_final[b] : (zd*)out[b];
-        ea.out32[b] = (cf*)ub[b];
+        ea.out[b] = (zd*)d[b];
+        ea.out32[b] = lo[b];
     }
     ea.a1 = a1;
-    ea.mw = u_final ? c->Mz : c->m16;
-    ea.mo = u_final ? c->Mz : c->m16;
-    ea.mu0 = c->m16;
-    ea.m32 = c->m16;
-    int st = run_nonlinear_passes<float, float>(c, ub, 1, c->m16, 0, d);
+    ea.mw = ea.mo = ea.mu0 = c->m16;
+    int st = run_nonlinear_passes<float, float>(c, hic, 1, c->m16, 0, first ? nullptr : dc);
     if (st) return st;
-    st = u_final ? run_x_forward_epilogue<float, EPI_INC_FINAL>(c, ea) : run_x_forward_epilogue<float, EPI_INC>(c, ea);
+    st = last ? run_x_forward_epilogue<float, EPI_INC_FINAL>(c, ea) : run_x_forward_epilogue<float, EPI_INC>(c, ea);
     if (st) return st;
     return hipGetLastError() == hipSuccess ? NS3D_OK : NS3D_ERR_HIP;
 }
 
-int rk3_one_incr(ns3d_ctx* c, zd* const u[3], const zd* const f[3]) {
-    const long nc = c->nc16;
-    cf* d1[3] = {c->s32 + 3 * nc, c->s32 + 4 * nc, c->s32 + 5 * nc};
-    cf* d2[3] = {c->s32 + 6 * nc, c->s32 + 7 * nc, c->s32 + 8 * nc};
-    const cf* d1c[3] = {d1[0], d1[1], d1[2]};
-    const cf* d2c[3] = {d2[0], d2[1], d2[2]};
+int rk3_one_incr(ns3d_ctx* c, const zd* const f[3]) {
     int st;
-    if ((st = stage_incr(c, nullptr, 1.0, d1, f, nullptr))) return st;
-    if ((st = stage_incr(c, d1c, 1.0 / 4.0, d2, f, nullptr))) return st;
-    return stage_incr(c, d2c, 2.0 / 3.0, nullptr, f, u);
+    if ((st = stage_incr(c, true, 1.0, false, f))) return st;
+    if ((st = stage_incr(c, false, 1.0 / 4.0, false, f))) return st;
+    return stage_incr(c, false, 2.0 / 3.0, true, f);
 }
 
 // the RK3 loops of NS3D_FP32_STATE and NS3D_FP32_INCR keep 9 FP32 fields (s32)
@@ -762,12 +764,17 @@ int rk3_steps(ns3d_ctx* c, int nsteps, zd* const u[3], const zd* const f[3]) {
         int st = ensure_state32(c);
         if (!st) st = ensure_scratch(c, 15);
         if (st) return st;
+        const long n16 = c->nc16;
         for (int b = 0; b < 3; b++)
-            hipLaunchKernelGGL(k_d2f_rows, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, u[b], c->s32 + b * c->nc16, c->nc,
-                               c->Mz, c->m16);
+            hipLaunchKernelGGL(k_split_rows, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, u[b], c->s32 + b * n16,
+                               c->s32 + (3 + b) * n16, c->nc, c->Mz, c->m16);
         HIPCHK(hipGetLastError());
         for (int i = 0; i < nsteps; i++)
-            if ((st = rk3_one_incr(c, u, f))) return st;
+            if ((st = rk3_one_incr(c, f))) return st;
+        for (int b = 0; b < 3; b++)
+            hipLaunchKernelGGL(k_join_rows, dim3(nblk(c->nc, 256)), dim3(256), 0, c->stream, c->s32 + b * n16,
+                               c->s32 + (3 + b) * n16, u[b], c->nc, c->Mz, c->m16);
+        HIPCHK(hipGetLastError());
         return NS3D_OK;
     }
     if (c->precision != NS3D_FP32_STATE) {
@@ -1245,7 +1252,7 @@ __global__ void k_tables(SpecTables tab, long nc, int Nx, int Ny, int Mz, double
     const int l = (int)(g % Mz);
     const long t1 = g / Mz;
     const int k = (int)(t1 % Ny), j = (int)(t1 / Ny);
-    const ProjCoef c = proj_coef(tab, j, k, l);
+    const ProjCoef c = proj_coef<true>(tab, j, k, l);
     if (din_diffusion) din_diffusion[g] = c.din;
     if (din_poisson) din_poisson[g] = g == 0 ? 1.0 : -c.din;
     if (a11) {

@@ -77,21 +77,35 @@ __device__ __forceinline__ void stage_twiddles(cplx<T>* __restrict__ dst, const 
     __syncthreads();
 }
 
-// Reference AM_ab and din_diffusion at (j,k,l).
+// Reference AM_ab and din_diffusion at (j,k,l).  EXACT: the six divisions of
+// the reference's host tables (Source/math_support.cu:418-466), bit for bit
+// (ns3d_tables); otherwise one reciprocal of the denominator times each
+// numerator (within an ulp of the table values; the fused epilogues).
 struct ProjCoef {
     double a11, a22, a33, a12, a13, a23, din;
 };
+template <bool EXACT = false>
 __device__ __forceinline__ ProjCoef proj_coef(const SpecTables& t, int j, int k, int l) {
     ProjCoef c;
     double den = (t.px[j] + t.py[k]) + t.pz[l];
     const bool zero = ((j | k | l) == 0);
     if (zero) den = 1.0;
-    c.a11 = t.px[j] / den;
-    c.a22 = t.py[k] / den;
-    c.a33 = t.pz[l] / den;
-    c.a12 = ((t.k1k2 * t.fm[j]) * t.fn[k]) / den;
-    c.a13 = ((t.k1k3 * t.fm[j]) * t.fq[l]) / den;
-    c.a23 = ((t.k2k3 * t.fn[k]) * t.fq[l]) / den;
+    if constexpr (EXACT) {
+        c.a11 = t.px[j] / den;
+        c.a22 = t.py[k] / den;
+        c.a33 = t.pz[l] / den;
+        c.a12 = ((t.k1k2 * t.fm[j]) * t.fn[k]) / den;
+        c.a13 = ((t.k1k3 * t.fm[j]) * t.fq[l]) / den;
+        c.a23 = ((t.k2k3 * t.fn[k]) * t.fq[l]) / den;
+    } else {
+        const double inv = 1.0 / den;
+        c.a11 = t.px[j] * inv;
+        c.a22 = t.py[k] * inv;
+        c.a33 = t.pz[l] * inv;
+        c.a12 = ((t.k1k2 * t.fm[j]) * t.fn[k]) * inv;
+        c.a13 = ((t.k1k3 * t.fm[j]) * t.fq[l]) * inv;
+        c.a23 = ((t.k2k3 * t.fn[k]) * t.fq[l]) * inv;
+    }
     if (zero) { c.a11 = c.a22 = c.a33 = c.a12 = c.a13 = c.a23 = 0.0; }
     c.din = (t.sx[j] + t.sy[k]) + t.sz[l];
     return c;

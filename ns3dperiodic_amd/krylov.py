@@ -461,7 +461,10 @@ def newton(ops, rhs, jacobian, x, tol=1e-10, max_iter=50, alpha=0.8, L=6, lin_to
     F(x); jacobian(x) returns a matvec v -> J(x) v for the current x.  Each
     step solves J delta = -F(x) by BiCGStab(L) and sets x += alpha*delta; stop
     once ||F(x)|| < tol after at least `min_iter` steps (Newton.cu:9-98).
-    Returns dict(converged, iterations, residual, history)."""
+    A breakdown of the linear solve (BiCGStab(L) flag < 0) ends the
+    iteration unconverged, x left at the last accepted step.
+    Returns dict(converged, iterations, residual, history[, breakdown: the
+    BiCGStab flag, lin_history: (iterations, relative residual) per solve])."""
     n = x.shape[-1]
     F = ops.zeros(n)
     delta = ops.zeros(n)
@@ -471,22 +474,25 @@ def newton(ops, rhs, jacobian, x, tol=1e-10, max_iter=50, alpha=0.8, L=6, lin_to
     if fn < tol:
         return {"converged": True, "iterations": 0, "residual": fn, "history": hist}
     it = 0
+    lin = []
     for it in range(1, max_iter + 1):
         ops.axpby(0.0, F, -1.0, F)              # b = -F(x)
         mv = jacobian(x)
         ops.axpby(0.0, F, 0.0, delta)
-        flag, _, _ = bicgstab_l(ops, mv, F, delta, L=L, tol=lin_tol, max_iter=lin_iter)
+        flag, lit, lres = bicgstab_l(ops, mv, F, delta, L=L, tol=lin_tol, max_iter=lin_iter)
+        lin.append((lit, lres))
         if flag < 0:
-            raise RuntimeError(f"Newton: linear solve breakdown (flag {flag}) at step {it}")
+            return {"converged": False, "iterations": it - 1, "residual": fn, "history": hist, "breakdown": flag,
+                    "lin_history": lin}
         ops.axpby(alpha, delta, 1.0, x)
         rhs(x, F)
         fn = math.sqrt(max(ops.dot(F, F), 0.0))
         hist.append(fn)
         if fn < tol and it >= min_iter:
-            return {"converged": True, "iterations": it, "residual": fn, "history": hist}
+            return {"converged": True, "iterations": it, "residual": fn, "history": hist, "lin_history": lin}
         if not math.isfinite(fn):
             break
-    return {"converged": False, "iterations": it, "residual": fn, "history": hist}
+    return {"converged": False, "iterations": it, "residual": fn, "history": hist, "lin_history": lin}
 
 
 # ----------------------------------------------------------------------------- batched FD J.v
@@ -667,9 +673,10 @@ class NSOperators:
       Source/NS3D_periodic.cu:444-470, which instead keeps advancing the base).
     """
 
-    def __init__(self, g, u_base=None, f=None):
+    def __init__(self, g, u_base=None, f=None, tau=None):
         self.g = g
         self.f = f
+        self.tau = tau
         self.n = g.vector_length_reduced()
         self.ops = GpuOps(g)
         if u_base is not None:
@@ -677,11 +684,15 @@ class NSOperators:
 
     def set_base(self, u):
         self.u = [c.clone() for c in u]
-        self.g.set_base_state(self.u, self.f)
+        self.g.set_base_state(self.u, self.f, tau=1.0 if self.tau is None else self.tau)
 
     def jv(self, vin, vout):
+        """J v, or with `tau` set the globalised J v - v/tau of Newton's
+        pseudo-time step (NSCallMatrixVector_reduced_Newton_Globalization,
+        Source/NS3D_periodic.cu:364-389; the reference's tau is 5)."""
         import ctypes as ct
-        self.g.lib.ns3d_jv_reduced(self.g.ctx, ct.c_void_p(vin.data_ptr()), ct.c_void_p(vout.data_ptr()))
+        fn = self.g.lib.ns3d_jv_reduced if self.tau is None else self.g.lib.ns3d_jv_reduced_newton
+        fn(self.g.ctx, ct.c_void_p(vin.data_ptr()), ct.c_void_p(vout.data_ptr()))
         from ._lib import check
         check(self.g.lib.ns3d_last_error(self.g.ctx), "ns3d_jv_reduced")
         return vout

@@ -28,7 +28,7 @@ def test_select_rule(tmp_path, monkeypatch):
     ms = {}
     sizes = {"pow2": 256, "r3": 192}
     for N in sizes.values():
-        for p, t in (("fp64", 4.70), ("fp32", 3.06), ("fp32s", 2.75)):
+        for p, t in (("fp64", 4.70), ("fp32", 3.06), ("fp32s", 2.75), ("fp32i", 2.90)):
             ms[f"base@{N}|{p}"] = t
     for n, o in vs[1:]:
         (k, v), = o.items()
@@ -38,8 +38,9 @@ def test_select_rule(tmp_path, monkeypatch):
     alt = next(c for c in autotune.SPACE["NS3D_PZT64"][1] if c != base["NS3D_PZT64"])
     ms[f"NS3D_PZT64={alt}@256|fp64"] = 4.60                          # ... but one (2% faster: taken)
     alt32 = next(c for c in autotune.SPACE["NS3D_PXFWD32"][1] if c != base["NS3D_PXFWD32"])
-    ms[f"NS3D_PXFWD32={alt32}@256|fp32"] = 3.05                      # 0.3% on the fp32+fp32s sum: kept out
+    ms[f"NS3D_PXFWD32={alt32}@256|fp32"] = 3.05                      # 0.3% on the fp32+fp32s+fp32i sum: kept out
     ms[f"NS3D_PXFWD32={alt32}@256|fp32s"] = 2.74
+    ms[f"NS3D_PXFWD32={alt32}@256|fp32i"] = 2.89
     alt3 = next(c for c in autotune.SPACE["NS3D_R3_PYINV64"][1] if c != base["NS3D_R3_PYINV64"])
     ms[f"NS3D_R3_PYINV64={alt3}@192|fp64"] = 4.50                    # radix-3 group, timed at 192^3
     json.dump({"N": 256, "N3": 192, "dt": {"pow2": 1.9e-3, "r3": 2.5e-3}, "steps": 20, "gpu": "test",
