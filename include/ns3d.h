@@ -179,6 +179,18 @@ int ns3d_rk3_ssp_graph(ns3d_ctx *ctx, int nsteps, ns3d_cplx *ux, ns3d_cplx *uy, 
 #define NS3D_MAX_BATCH 1024
 int ns3d_rk3_ssp_batch(ns3d_ctx *ctx, int nbatch, int nsteps, ns3d_cplx *u,
                        const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz);
+/* ns3d_rk3_ssp_batch with per-member parameters, for a batch of resident requests of one grid:
+ * dt, Re (device arrays of nbatch doubles) and steps (device array of nbatch ints): member b takes
+ * the first steps[b] of the nsteps steps (0: an idle slot, untouched) with its own dt and Re; its
+ * blocks of the other steps return at once.  The arrays are read when the kernels run, so a caller
+ * may rewrite them between calls.  graph != 0 replays a HIP graph of the nsteps steps captured on
+ * the first call (and again whenever the pointers, nbatch or nsteps change; needs a created
+ * stream).  Each member's result is bitwise what ns3d_rk3_ssp_steps gives it alone with that dt
+ * and Re.  NS3D_FP64 and NS3D_FP32_BULK (NS3D_ERR_PRECISION otherwise).  No reference counterpart:
+ * the batching of a request server (one launch per pass for every resident request of a grid). */
+int ns3d_rk3_ssp_batch_members(ns3d_ctx *ctx, int nbatch, int nsteps, ns3d_cplx *u,
+                               const ns3d_cplx *fx, const ns3d_cplx *fy, const ns3d_cplx *fz,
+                               const double *dt, const double *Re, const int *steps, int graph);
 
 /* RK3_SSP of the velocity plus a passive scalar theta (physical, double [Nx][Ny][Nz]) advected
  * with WENO (scheme 3/5) by each stage's 2/3-masked physical velocity, with the same SSP

@@ -66,6 +66,7 @@ SIGNATURES = {
     "ns3d_rk3_ssp_scalar": (I, [P, I] + [P] * 7 + [I]),
     "ns3d_rk3_ssp_graph": (I, [P, I] + [P] * 6),
     "ns3d_rk3_ssp_batch": (I, [P, I, I] + [P] * 4),
+    "ns3d_rk3_ssp_batch_members": (I, [P, I, I] + [P] * 7 + [I]),
     "ns3d_kinetic_energy_spectral": (I, [P] + [P] * 4),
     "ns3d_dissipation_spectral": (I, [P] + [P] * 4),
     "ns3d_divergence": (I, [P] + [P] * 4),

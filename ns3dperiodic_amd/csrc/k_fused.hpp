@@ -55,6 +55,16 @@ template <int N, int E> using XYPlan = typename XYSel<N, E>::type;
 // member stride it is given (bsc: scratch, bwi: input state; 0 when z is 1).
 #define NS3D_FIELD_LOOP(f, n) for (int f = blockIdx.y; f < (int)blockIdx.y + 1; f++)
 
+// Members of a batch that sit out a step (resident requests of one grid that
+// need fewer steps than the launch carries): member z takes part in step
+// `step` iff step < left[z]; left null: every member does.  The test is
+// uniform over a block, so an idle member's blocks return before any barrier.
+struct MemberMask {
+    const int* left;
+    int step;
+};
+__device__ __forceinline__ bool member_idle(const MemberMask& m) { return m.left && m.step >= m.left[blockIdx.z]; }
+
 // ---------------------------------------------------------------- P1
 // x-inverse of M w^_b and of M (i kx w^_b).  Block: one y index k and P
 // consecutive z indices l; lanes p-fastest so each row load is contiguous.
@@ -67,13 +77,13 @@ template <typename TB, int NX, int P, int NF, int E, int OS, typename TS = doubl
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
 k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc, int Ny, int Mz, int Mw, int Ms,
              int nlt, int nb, SpecTables tab, const cplx<TB>* __restrict__ twg, long bwi = 0, long bsc = 0,
-             PtrArr<const cplx<TS>, 3> w2 = {}) {
+             PtrArr<const cplx<TS>, 3> w2 = {}, MemberMask mm = {}) {
     constexpr int TPP = XYPlan<NX, E>::TPP;
     using F = PencilFFT<TB, NX, TPP, +1, typename XYPlan<NX, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NX * P];
     __shared__ cplx<TB> tw[NX];
     const int bid = xcd_tile();
-    if (bid >= nb) return;
+    if (bid >= nb || member_idle(mm)) return;
     const int tid = threadIdx.x, p = tid % P, t = tid / P;
     const int k = bid / nlt, lt = bid % nlt;
     const int l = lt * P + p;
@@ -141,13 +151,13 @@ k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc
 template <typename TB, int NY, int P, int NF, int E, int OS>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NY, E>::TPP))
 k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, int Ny, int Mz, int Ms, int nlt,
-             int nb, SpecTables tab, const cplx<TB>* __restrict__ twg, long bsc = 0) {
+             int nb, SpecTables tab, const cplx<TB>* __restrict__ twg, long bsc = 0, MemberMask mm = {}) {
     constexpr int TPP = XYPlan<NY, E>::TPP;
     using F = PencilFFT<TB, NY, TPP, +1, typename XYPlan<NY, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NY * P];
     __shared__ cplx<TB> tw[NY];
     const int bid = xcd_tile();
-    if (bid >= nb) return;
+    if (bid >= nb || member_idle(mm)) return;
     const int tid = threadIdx.x, p = tid % P, t = tid / P;
     const int j = bid / nlt, lt = bid % nlt;
     if (lt * P > tab.lmax) return;  // z-planes beyond lmax are zero; the z pass skips them
@@ -471,7 +481,8 @@ template <typename TB, int NZ, int P, int NF>
 __global__ void __attribute__((amdgpu_waves_per_eu(NF == 1 ? (sizeof(TB) == 4 ? NS3D_WAVES_Z32 : NS3D_WAVES_Z) : NS3D_WAVES_Z2, 8)))
 __launch_bounds__((P * ZPlanT<TB, NZ>::TPP))
 k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, long npen, int Mz, int nb,
-            SpecTables tab, const cplx<TB>* __restrict__ twg, TB scale, TB* __restrict__ uphys, long nr, long bsc = 0) {
+            SpecTables tab, const cplx<TB>* __restrict__ twg, TB scale, TB* __restrict__ uphys, long nr, long bsc = 0,
+            MemberMask mm = {}) {
     // B is zero on z-planes l > lmax (never written by the y pass): TRUNC skips them
     using Z = ZPencil<TB, NZ, P, true>;
     constexpr int TPP = Z::TPP, NPT = Z::NPT;
@@ -488,7 +499,7 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
     __shared__ cplx<TB> lds[(ILP ? 2 : 1) * P * (NZ + Z::PAD)];
     __shared__ cplx<TB> tw[NZ];
     const int bid = xcd_tile();
-    if (bid >= nb) return;
+    if (bid >= nb || member_idle(mm)) return;
     stage_twiddles<TB, NZ>(tw, twg);
     const int tid = threadIdx.x, t = tid % TPP, p = tid / TPP;
     const long pen = (long)bid * P + p;
@@ -651,13 +662,13 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
 template <typename TB, int NA, int P, int DIR, int AXIS, int E>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NA, E>::TPP))
 k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, int Ms, int nlt, int nb,
-           const cplx<TB>* __restrict__ twg, long bsc = 0) {
+           const cplx<TB>* __restrict__ twg, long bsc = 0, MemberMask mm = {}) {
     constexpr int TPP = XYPlan<NA, E>::TPP;
     using F = PencilFFT<TB, NA, TPP, DIR, typename XYPlan<NA, E>::R, IndexMajor<P>>;
     __shared__ cplx<TB> lds[NA * P];
     __shared__ cplx<TB> tw[NA];
     const int bid = xcd_tile();
-    if (bid >= nb) return;
+    if (bid >= nb || member_idle(mm)) return;
     stage_twiddles<TB, NA>(tw, twg);
     const int tid = threadIdx.x, p = tid % P, t = tid / P;
     const int o = bid / nlt, lt = bid % nlt;
@@ -720,6 +731,9 @@ struct EpiArgs {
     cplx<double>* out[3];        // output (EPI_Q, EPI_STAGE, EPI_RHS); cplx<float> for EPI_STAGE32
     double dt, Re, a0, a1;       // EPI_STAGE: out = a0*u0 + a1*E(w)
     double nu, dtnu;             // 1/Re and dt/Re (formed once on the host)
+    // per-member dt and Re of a batch (device arrays indexed by blockIdx.z; null: dt, Re above)
+    const double* dtv;
+    const double* rev;
     int ipq;                     // reference IPQ rule (Q1)
     // mode (j,k,l) of w / u0 / out / f sits at (j*ny + k)*m + l: m = Mz for the
     // reference layout, a padded row stride for the library's own stage buffers
@@ -872,14 +886,14 @@ __device__ __forceinline__ void ipq_rule(cplx<double>& Qx, cplx<double>& Qy, cpl
 template <typename TB, int NX, int P, int MODE, int E>
 __global__ void __attribute__((amdgpu_waves_per_eu(sizeof(TB) == 4 ? NS3D_WAVES_XFWD32 : NS3D_WAVES_XY, 8))) __launch_bounds__((P * XYPlan<NX, E>::TPP))
 k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int nlt, int nb, SpecTables tab,
-            const cplx<TB>* __restrict__ twg, EpiArgs ea, long bsc = 0) {
+            const cplx<TB>* __restrict__ twg, EpiArgs ea, long bsc = 0, MemberMask mm = {}) {
     constexpr int TPP = XYPlan<NX, E>::TPP;
     using F = PencilFFT<TB, NX, TPP, -1, typename XYPlan<NX, E>::R, IndexMajor<P>>;
     constexpr bool ILP = NS3D_HOIST && (NS3D_XFWD_ILP == 2 || (NS3D_XFWD_ILP == 1 && sizeof(TB) == 8));
     __shared__ cplx<TB> lds[(ILP ? 2 : 1) * NX * P];
     __shared__ cplx<TB> tw[NX];
     const int bid = xcd_tile();
-    if (bid >= nb) return;
+    if (bid >= nb || member_idle(mm)) return;
     stage_twiddles<TB, NX>(tw, twg);
     const int tid = threadIdx.x, p = tid % P, t = tid / P;
     const int k = bid / nlt, lt = bid % nlt;
@@ -887,6 +901,13 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int
     const bool lv = l < Mz;
     const long Si = (long)Ny * Ms;       // D: rows padded to Ms
     const long basei = (long)k * Ms + l;
+    if (ea.dtv) {                        // per-member time step and Reynolds number
+        const double dtm = ea.dtv[blockIdx.z], rem = ea.rev[blockIdx.z];
+        ea.dt = dtm;
+        ea.Re = rem;
+        ea.nu = 1.0 / rem;
+        ea.dtnu = dtm / rem;
+    }
     if (blockIdx.z) {                    // batch member z
         const long z = blockIdx.z;
         D += z * bsc;

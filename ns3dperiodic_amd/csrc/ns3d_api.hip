@@ -220,6 +220,15 @@ struct ns3d_ctx {
     long bsc = 0;
     zd* bstage = nullptr;  // batch members' u1[3], u2[3]: 6 fields of nc8 each per member
     int bstage_members = 0;
+    // members sitting out steps, and per-member dt / Re (ns3d_rk3_ssp_batch_members)
+    MemberMask mm = {nullptr, 0};
+    const double* mdt = nullptr;
+    const double* mre = nullptr;
+    // graph of ns3d_rk3_ssp_batch_members for fixed pointers, nbatch and nsteps
+    hipGraph_t mgraph = nullptr;
+    hipGraphExec_t mgraph_exec = nullptr;
+    const void* mgraph_key[8] = {};
+    int mgraph_nb = -1, mgraph_steps = -1;
     // optional per-pass timing with HIP events on the context stream
     bool prof_on = false;
     std::vector<hipEvent_t> ev_pool;
@@ -395,6 +404,11 @@ void drop_graph(ns3d_ctx* c) {
     c->graph_exec = nullptr;
     c->graph = nullptr;
     c->graph_steps = -1;
+    if (c->mgraph_exec) hipGraphExecDestroy(c->mgraph_exec);
+    if (c->mgraph) hipGraphDestroy(c->mgraph);
+    c->mgraph_exec = nullptr;
+    c->mgraph = nullptr;
+    c->mgraph_nb = c->mgraph_steps = -1;
 }
 
 int ensure_scratch(ns3d_ctx* c, int fields) {
@@ -465,15 +479,15 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
             PtrArr<const cplx<TS>, 3> w{{in[0], in[1], in[2]}};
             PtrArr<const cplx<TS>, 3> w2{{in2[0], in2[1], in2[2]}};
             hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E, OS, TS, 1>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, mw, Mi, nlt, nb,
-                               c->tab, twp<TB>(c, 0), bin, bsc, w2);
+                               c->tab, twp<TB>(c, 0), bin, bsc, w2, c->mm);
         } else if (nf == 1) {
             PtrArr<const cplx<TS>, 3> w{{in[0], in[1], in[2]}};
             hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 1, E, OS, TS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, mw, Mi, nlt, nb, c->tab,
-                               twp<TB>(c, 0), bin, bsc);
+                               twp<TB>(c, 0), bin, bsc, PtrArr<const cplx<TS>, 3>{}, c->mm);
         } else if constexpr (std::is_same<TS, double>::value) {
             PtrArr<const zd, 6> w{{in[0], in[1], in[2], in[3], in[4], in[5]}};
             hipLaunchKernelGGL((k_x_inv_grad<TB, NX, P, 2, E, OS>), grid, blk, 0, s, w, A, nc, c->Ny, c->Mz, mw, Mi, nlt, nb, c->tab,
-                               twp<TB>(c, 0), bin, bsc);
+                               twp<TB>(c, 0), bin, bsc, PtrArr<const zd, 3>{}, c->mm);
         } else {
             return NS3D_ERR_ARG;
         }
@@ -494,10 +508,10 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
         dim3 grid = xgrid(nb, 3 * nf * (OS ? 3 : 1), nz), blk(P * XYPlan<NY, E>::TPP);
         if (nf == 1)
             hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 1, E, OS>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, Mi, nlt, nb, c->tab,
-                               twp<TB>(c, 1), bsc);
+                               twp<TB>(c, 1), bsc, c->mm);
         else
             hipLaunchKernelGGL((k_y_inv_grad<TB, NY, P, 2, E, OS>), grid, blk, 0, s, A, B, nc, c->Ny, c->Mz, Mi, nlt, nb, c->tab,
-                               twp<TB>(c, 1), bsc);
+                               twp<TB>(c, 1), bsc, c->mm);
         return NS3D_OK;
     });
     prof_end(c);
@@ -513,10 +527,10 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
         TB* up = nf == 1 ? (TB*)c->uphys_out : nullptr;
         if (nf == 1)
             hipLaunchKernelGGL((k_z_product<TB, NZ, P, 1>), grid, blk, 0, s, B, A, nc, c->npen, Mi, nb, c->tab, twp<TB>(c, 2), scale,
-                               up, c->nr, bsc);
+                               up, c->nr, bsc, c->mm);
         else
             hipLaunchKernelGGL((k_z_product<TB, NZ, P, 2>), grid, blk, 0, s, B, A, nc, c->npen, Mi, nb, c->tab, twp<TB>(c, 2), scale,
-                               (TB*)nullptr, c->nr, bsc);
+                               (TB*)nullptr, c->nr, bsc, c->mm);
         return NS3D_OK;
     });
     prof_end(c);
@@ -531,7 +545,7 @@ int run_nonlinear_passes(ns3d_ctx* c, const cplx<TS>* const* in, int nf, int mw 
         const int nb = c->Nx * nlt;
         dim3 grid = xgrid(nb, 3, nz), blk(P * XYPlan<NY, E>::TPP);
         hipLaunchKernelGGL((k_axis_c2c<TB, NY, P, -1, 1, E>), grid, blk, 0, s, A, A + 3 * nc, 3, nc, c->Ny, c->Mz, Mi, nlt, nb,
-                           twp<TB>(c, 1), bsc);
+                           twp<TB>(c, 1), bsc, c->mm);
         return NS3D_OK;
     });
     prof_end(c);
@@ -550,7 +564,7 @@ int run_x_forward_epilogue(ns3d_ctx* c, const EpiArgs& ea) {
         const int nb = c->Ny * nlt;
         dim3 grid = xgrid(nb, 1, c->batch), blk(P * XYPlan<NX, E>::TPP);
         hipLaunchKernelGGL((k_x_fwd_epi<TB, NX, P, MODE, E>), grid, blk, 0, c->stream, D, c->nci, c->Ny, c->Mz, mi_of(c), nlt, nb,
-                           c->tab, twp<TB>(c, 0), ea, c->bsc);
+                           c->tab, twp<TB>(c, 0), ea, c->bsc, c->mm);
         return NS3D_OK;
     });
     prof_end(c);
@@ -570,6 +584,8 @@ EpiArgs make_epi(const ns3d_ctx* c) {
     ea.ny = c->Ny;
     ea.mw = ea.mu0 = ea.mo = ea.mf = c->Mz;
     ea.nf = c->nf;
+    ea.dtv = c->mdt;
+    ea.rev = c->mre;
     return ea;
 }
 
@@ -724,8 +740,7 @@ bool uses_s32(const ns3d_ctx* c) { return c->precision == NS3D_FP32_STATE || c->
 // component k at u + (3b + k)*nc (reference layout); forcing shared.  The
 // kernels and their order are those of rk3_one, so each member's result is
 // bitwise the one rk3_one gives it alone.
-int rk3_one_batch(ns3d_ctx* c, int nb, zd* u, const zd* const f[3]) {
-    const long n8 = c->nc8, nc = c->nc;
+int ensure_batch(ns3d_ctx* c, int nb) {
     if (c->bstage_members < nb) {
         drop_graph(c);
         if (c->bstage) {
@@ -735,10 +750,15 @@ int rk3_one_batch(ns3d_ctx* c, int nb, zd* u, const zd* const f[3]) {
             c->bstage = nullptr;
             c->bstage_members = 0;
         }
-        if (hipMalloc((void**)&c->bstage, sizeof(zd) * (size_t)n8 * 6 * nb) != hipSuccess) return NS3D_ERR_ALLOC;
+        if (hipMalloc((void**)&c->bstage, sizeof(zd) * (size_t)c->nc8 * 6 * nb) != hipSuccess) return NS3D_ERR_ALLOC;
         c->bstage_members = nb;
     }
-    int st = ensure_scratch(c, 15 * nb);
+    return ensure_scratch(c, 15 * nb);
+}
+
+int rk3_one_batch(ns3d_ctx* c, int nb, zd* u, const zd* const f[3]) {
+    const long n8 = c->nc8, nc = c->nc;
+    int st = ensure_batch(c, nb);
     if (st) return st;
     zd* u1[3] = {c->bstage, c->bstage + n8, c->bstage + 2 * n8};
     zd* u2[3] = {c->bstage + 3 * n8, c->bstage + 4 * n8, c->bstage + 5 * n8};
@@ -755,6 +775,24 @@ int rk3_one_batch(ns3d_ctx* c, int nb, zd* u, const zd* const f[3]) {
             st = stage_into(c, u2c, uc, 1.0 / 3.0, 2.0 / 3.0, uu, f, m8, mz, mz, bst, bs, bs);
     c->batch = 1;
     c->bsc = 0;
+    return st;
+}
+
+// nsteps RK3 steps of nb batch members with per-member dt, Re (device arrays)
+// and step counts: member b takes the first left[b] steps and sits out the rest.
+int rk3_batch_members(ns3d_ctx* c, int nb, int nsteps, zd* u, const zd* const f[3], const double* dt, const double* re,
+                      const int* left) {
+    c->mm.left = left;
+    c->mdt = dt;
+    c->mre = re;
+    int st = NS3D_OK;
+    for (int i = 0; i < nsteps && !st; i++) {
+        c->mm.step = i;
+        st = rk3_one_batch(c, nb, u, f);
+    }
+    c->mm.left = nullptr;
+    c->mm.step = 0;
+    c->mdt = c->mre = nullptr;
     return st;
 }
 
@@ -1183,8 +1221,7 @@ int ns3d_destroy(ns3d_ctx* c) {
     hipFree(c->pb);
     if (c->pb_host) hipHostFree(c->pb_host);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
-    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) hipGraphDestroy(c->graph);
+    drop_graph(c);
     delete c;
     return NS3D_OK;
 }
@@ -1439,6 +1476,49 @@ int ns3d_rk3_ssp_batch(ns3d_ctx* c, int nbatch, int nsteps, ns3d_cplx* u, const 
         int st = rk3_one_batch(c, nbatch, ub, f);
         if (st) return st;
     }
+    return checked(c, NS3D_OK);
+}
+
+int ns3d_rk3_ssp_batch_members(ns3d_ctx* c, int nbatch, int nsteps, ns3d_cplx* u, const ns3d_cplx* fx,
+                               const ns3d_cplx* fy, const ns3d_cplx* fz, const double* dt, const double* Re,
+                               const int* steps, int graph) {
+    if (!c || !u || !dt || !Re || !steps || nbatch < 1 || nsteps < 0) return NS3D_ERR_ARG;
+    if (nbatch > NS3D_MAX_BATCH) return NS3D_ERR_SIZE;
+    if (c->precision != NS3D_FP64 && c->precision != NS3D_FP32_BULK) return NS3D_ERR_PRECISION;
+    const zd* f[3] = {(const zd*)fx, (const zd*)fy, (const zd*)fz};
+    if (!graph) return checked(c, rk3_batch_members(c, nbatch, nsteps, (zd*)u, f, dt, Re, steps));
+    if (c->stream == nullptr || nsteps < 1) return NS3D_ERR_ARG;  // capture needs a created stream
+    const void* key[8] = {u, fx, fy, fz, dt, Re, steps, nullptr};
+    bool same = c->mgraph_exec && c->mgraph_nb == nbatch && c->mgraph_steps == nsteps;
+    for (int i = 0; same && i < 8; i++) same = key[i] == c->mgraph_key[i];
+    if (!same) {
+        drop_graph(c);
+        int st = ensure_batch(c, nbatch);  // every buffer the steps use exists before capture
+        if (st) return st;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const bool prof = c->prof_on;
+        c->prof_on = false;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        st = rk3_batch_members(c, nbatch, nsteps, (zd*)u, f, dt, Re, steps);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        c->prof_on = prof;
+        if (st || e != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return st ? st : NS3D_ERR_HIP;
+        }
+        c->mgraph = g;
+        if (hipGraphInstantiate(&c->mgraph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            hipGraphDestroy(g);
+            c->mgraph = nullptr;
+            c->mgraph_exec = nullptr;
+            return NS3D_ERR_HIP;
+        }
+        for (int i = 0; i < 8; i++) c->mgraph_key[i] = key[i];
+        c->mgraph_nb = nbatch;
+        c->mgraph_steps = nsteps;
+    }
+    HIPCHK(hipGraphLaunch(c->mgraph_exec, c->stream));
     return checked(c, NS3D_OK);
 }
 

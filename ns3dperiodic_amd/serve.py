@@ -179,6 +179,15 @@ class Ledger:
         self.active -= 1
         assert self.used >= 0 and self.active >= 0
 
+    def reserve(self, nbytes):
+        """Memory held without a request of its own (a batching group's slots)."""
+        self.used += nbytes
+        self.peak = max(self.peak, self.used)
+
+    def unreserve(self, nbytes):
+        self.used -= nbytes
+        assert self.used >= 0
+
 
 def admit_order(queue, ledger, cost, max_overtakes=2):
     """Requests of the FIFO `queue` to admit now.  Requests are taken in
@@ -306,12 +315,116 @@ class _Slot:
         self.done_ev.record(self.stream)
 
 
+def group_member_bytes(N, precision):
+    """HBM of one member slot of a batching group (its share of the batch):
+    3 FP64 state fields (reference layout), 6 FP64 RK stage fields (rows padded
+    to 8 modes) and 15 scratch fields in the bulk precision (padded rows)."""
+    nc = N * N * (N // 2 + 1)
+    bulk, pad = (16, 8) if precision == "fp64" else (8, 16)
+    return 16 * 3 * nc + 16 * 6 * _rows(N, 8) + bulk * 15 * _rows(N, pad)
+
+
+class _Group:
+    """Resident integrate requests of one grid size advanced together: a batch
+    of `cap` member slots on one context and stream, one launch per pass for
+    every member (ns3d_rk3_ssp_batch_members, per-member dt and Re).  Each
+    chunk of K steps gives every member min(K, steps left) steps; idle slots
+    and members that have finished sit the chunk out.  A member's result is
+    bitwise what its steps give it alone."""
+
+    def __init__(self, N, cap, precision, device, chunk, use_graphs, ipq, inflight):
+        import torch
+        from .solver import NS3D
+        self.N, self.cap, self.chunk, self.use_graphs = N, cap, chunk, use_graphs
+        self.member_bytes = group_member_bytes(N, precision)
+        self.bytes = cap * self.member_bytes + 16 * N * N * (N // 2 + 1)     # + the shared forcing
+        self.stream = torch.cuda.Stream(device=device)
+        self.g = NS3D(N, precision=precision, stream=self.stream, device=device, ipq=ipq, mask_nyquist=not ipq)
+        dev = torch.device("cuda", device)
+        cs = (N, N, N // 2 + 1)
+        self.ub = torch.zeros((cap, 3) + cs, dtype=torch.complex128, device=dev)
+        self.f = [torch.zeros(cs, dtype=torch.complex128, device=dev), None, None]
+        self.f_loaded = False
+        self.dt_d = torch.zeros(cap, dtype=torch.float64, device=dev)
+        self.re_d = torch.ones(cap, dtype=torch.float64, device=dev)
+        self.left_d = torch.zeros(cap, dtype=torch.int32, device=dev)
+        # host copies of the member parameters, uploaded with every chunk from a ring of pinned buffers
+        self.dt_h = np.zeros(cap)
+        self.re_h = np.ones(cap)
+        self.ring = [(torch.zeros(cap, dtype=torch.float64).pin_memory(), torch.zeros(cap, dtype=torch.float64).pin_memory(),
+                      torch.zeros(cap, dtype=torch.int32).pin_memory(), None) for _ in range(inflight + 1)]
+        self.ring_i = 0
+        self.members = [None] * cap           # slot -> dict(req, io, rem, admitted, done_ev)
+        self.inflight = collections.deque()
+
+    def free_slot(self):
+        return next((m for m in range(self.cap) if self.members[m] is None), None)
+
+    def active(self):
+        return sum(1 for m in self.members if m is not None)
+
+    def load(self, m, req, io, admitted):
+        import torch
+        with torch.cuda.stream(self.stream):
+            for b in range(3):
+                self.ub[m, b].copy_(io["u"][b], non_blocking=True)
+            if not self.f_loaded:             # every request of a grid has the forcing f_x = sin 4y
+                self.f[0].copy_(io["f"], non_blocking=True)
+                self.f_loaded = True
+        self.dt_h[m], self.re_h[m] = req.dt, req.Re
+        self.members[m] = {"req": req, "io": io, "rem": req.nsteps, "admitted": admitted, "done_ev": None}
+
+    def issue(self):
+        """Enqueue one chunk for every member with steps left; False if none has."""
+        import torch
+        left = np.zeros(self.cap, dtype=np.int32)
+        for m, mem in enumerate(self.members):
+            if mem is not None and mem["rem"] > 0:
+                left[m] = min(mem["rem"], self.chunk)
+        if not left.any():
+            return False
+        dt_p, re_p, left_p, ev = self.ring[self.ring_i]
+        if ev is not None:
+            ev.synchronize()                  # the upload that last used this host buffer is done
+        dt_p.numpy()[:] = self.dt_h
+        re_p.numpy()[:] = self.re_h
+        left_p.numpy()[:] = left
+        with torch.cuda.stream(self.stream):
+            self.dt_d.copy_(dt_p, non_blocking=True)
+            self.re_d.copy_(re_p, non_blocking=True)
+            self.left_d.copy_(left_p, non_blocking=True)
+            up = torch.cuda.Event()
+            up.record(self.stream)
+            self.ring[self.ring_i] = (dt_p, re_p, left_p, up)
+            self.ring_i = (self.ring_i + 1) % len(self.ring)
+            self.g.RK3_SSP_batch_members(self.ub, self.f, self.dt_d, self.re_d, self.left_d, self.chunk,
+                                         graph=self.use_graphs)
+            for m, mem in enumerate(self.members):
+                if left[m]:
+                    mem["rem"] -= int(left[m])
+                    if mem["rem"] == 0:       # its last chunk: download right behind it on this stream
+                        for h, b in zip(mem["io"]["out_u"], range(3)):
+                            h.copy_(self.ub[m, b], non_blocking=True)
+                        mem["done_ev"] = torch.cuda.Event()
+                        mem["done_ev"].record(self.stream)
+            ev = torch.cuda.Event()
+            ev.record(self.stream)
+        self.inflight.append(ev)
+        return True
+
+    def close(self):
+        self.g.close()
+
+
 class Server:
-    """Continuous-batching server: up to `max_active` resident requests, each
-    on its own slot (context + stream), admitted by HBM budget."""
+    """Continuous-batching server: up to `max_active` resident requests,
+    admitted by HBM budget.  With `batch` (default), integrate requests of one
+    grid size share a batching group (one launch per pass for all of them);
+    J.v and return-map requests, and every request without `batch`, run on
+    slots of their own (context + stream)."""
 
     def __init__(self, precision="fp32", max_active=8, budget_bytes=64 << 30, chunk=16, use_graphs=True,
-                 device=0, inflight=2, ipq=False):
+                 device=0, inflight=2, ipq=False, batch=True):
         self.precision = precision
         self.ipq = ipq
         self.ledger = Ledger(budget_bytes, max_active)
@@ -319,8 +432,10 @@ class Server:
         self.use_graphs = use_graphs
         self.device = device
         self.inflight = inflight
+        self.batch = batch and precision in ("fp64", "fp32")
         self.idle = collections.defaultdict(list)     # (N, kind) -> idle slots (context, graph, buffers kept)
         self.idle_bytes = 0
+        self.groups = {}                              # N -> _Group
 
     def _slot(self, r):
         key = (r.N, r.kind)
@@ -331,12 +446,61 @@ class Server:
         return _Slot(r.N, r.kind, self.precision, self.device, self.chunk, self.use_graphs, self.ipq)
 
     def _make_room(self, need):
-        # drop idle slots (largest first) while the budget is short
+        # drop idle slots (largest first), then empty groups, while the budget is short
         for key in sorted(self.idle, key=lambda k: -k[0]):
             while self.idle[key] and self.ledger.used + self.idle_bytes + need > self.ledger.budget:
                 s = self.idle[key].pop()
                 s.g.close()
                 self.idle_bytes -= s.bytes
+        for N in sorted(self.groups, reverse=True):
+            g = self.groups[N]
+            if g.active() == 0 and self.ledger.used + self.idle_bytes + need > self.ledger.budget:
+                g.close()
+                self.idle_bytes -= g.bytes
+                del self.groups[N]
+
+    def _batched(self, r):
+        return self.batch and r.kind == "integrate"
+
+    def _cost(self, r):
+        """HBM a request adds when admitted: its own bytes on a slot; on a
+        batching group, nothing if the group has members and a free member
+        slot, the group's memory if it is idle, else the memory of a new
+        one-member group.  A group holds its members' memory from its
+        creation: charged to the ledger as a reservation while it has members,
+        counted with the idle slots (reclaimable) while it has none."""
+        if not self._batched(r):
+            return request_bytes(r, self.precision)
+        g = self.groups.get(r.N)
+        if g is not None and g.free_slot() is not None:
+            return 0 if g.active() else g.bytes
+        return group_member_bytes(r.N, self.precision) + 16 * r.N * r.N * (r.N // 2 + 1)
+
+    def _admit_batched(self, r, now):
+        """Put r in its grid's group (created with as many member slots as
+        the budget and the slot limit allow); False if that is not possible now."""
+        g = self.groups.get(r.N)
+        if g is None:
+            mb = group_member_bytes(r.N, self.precision)
+            fb = 16 * r.N * r.N * (r.N // 2 + 1)
+            self._make_room(mb + fb)
+            room = self.ledger.budget - self.ledger.used - self.idle_bytes - fb
+            if room < mb:
+                return False
+            # at most half of the free memory beyond the first member, so other grids keep room
+            cap = min(self.ledger.max_active, max(1, (room // 2) // mb))
+            g = self.groups[r.N] = _Group(r.N, int(cap), self.precision, self.device, self.chunk, self.use_graphs,
+                                          self.ipq, self.inflight)
+            self.idle_bytes += g.bytes
+        m = g.free_slot()
+        if m is None:
+            return False
+        if g.active() == 0:                   # the group's memory moves from idle to reserved
+            self.idle_bytes -= g.bytes
+            self.ledger.reserve(g.bytes)
+        self.ledger.admit(0)
+        g.load(m, r, self._io[r.rid], now)
+        return True
 
     def run(self, requests, io, clock=time.perf_counter):
         """Serve `requests`; io[rid] holds the request's pinned host buffers
@@ -346,19 +510,26 @@ class Server:
         queue = collections.deque()
         active = []
         results = {}
-        cost = lambda r: request_bytes(r, self.precision)   # noqa: E731
+        cost = self._cost
+        self._io = io
         for r in requests:
-            if cost(r) > self.ledger.budget:
+            if request_bytes(r, self.precision) > self.ledger.budget:
                 raise ValueError(f"request {r.rid} needs {cost(r)} bytes, more than the whole budget")
             if r.nsteps < 1:
                 raise ValueError(f"request {r.rid} has no steps (request_inputs sets them)")
         t0 = clock()
-        while pending or queue or active:
+        while pending or queue or active or any(g.active() for g in self.groups.values()):
             now = clock() - t0
             while pending and pending[0].arrival <= now:
                 queue.append(pending.popleft())
             for r in admit_order(queue, self.ledger, cost):
+                if self._batched(r):
+                    if self._admit_batched(r, clock() - t0):
+                        queue.remove(r)
+                    continue                  # not possible now (group full or no room): stays queued
                 self._make_room(cost(r))
+                if not self.ledger.fits(cost(r)):
+                    continue                  # a group created in this pass took the room: stays queued
                 queue.remove(r)
                 self.ledger.admit(cost(r))
                 s = self._slot(r)
@@ -366,6 +537,23 @@ class Server:
                 s.admitted = clock() - t0
                 active.append(s)
             progressed = False
+            for g in self.groups.values():
+                while g.inflight and g.inflight[0].query():
+                    g.inflight.popleft()
+                while len(g.inflight) < self.inflight and g.issue():
+                    progressed = True
+                for m, mem in enumerate(g.members):
+                    if mem is not None and mem["done_ev"] is not None and mem["done_ev"].query():
+                        t = clock() - t0
+                        r = mem["req"]
+                        results[r.rid] = Result(r.rid, mem["admitted"], t, t - r.arrival,
+                                                [c.numpy() for c in mem["io"]["out_u"]], None, None)
+                        g.members[m] = None
+                        self.ledger.release(0)
+                        if g.active() == 0:   # idle group: its memory becomes reclaimable
+                            self.ledger.unreserve(g.bytes)
+                            self.idle_bytes += g.bytes
+                        progressed = True
             for s in list(active):
                 while s.inflight and s.inflight[0].query():
                     s.inflight.popleft()
@@ -382,7 +570,7 @@ class Server:
                     s.done_ev = None
                     s.req = s.io = None
                     active.remove(s)
-                    self.ledger.release(cost(r))
+                    self.ledger.release(request_bytes(r, self.precision))
                     self.idle[(r.N, r.kind)].append(s)
                     self.idle_bytes += s.bytes
                     progressed = True
@@ -396,6 +584,13 @@ class Server:
             for s in lst:
                 s.g.close()
         self.idle.clear()
+        for g in self.groups.values():
+            g.close()
+            if g.active():
+                self.ledger.unreserve(g.bytes)
+            else:
+                self.idle_bytes -= g.bytes
+        self.groups.clear()
 
 
 def pin_io(u, fx, v):

@@ -206,6 +206,18 @@ class NS3D:
               "ns3d_rk3_ssp_batch")
         return ub
 
+    def RK3_SSP_batch_members(self, ub, f, dt, Re, steps, nsteps, graph=False):
+        """nsteps RK3 steps of the members of ub ([B, 3, Nx, Ny, Mz] complex128) with per-member
+        dt, Re (float64 device tensors [B]) and step counts (int32 device tensor [B]: member b takes
+        the first steps[b] steps, 0 = idle); ns3d_rk3_ssp_batch_members, optionally graph-replayed."""
+        assert ub.dtype == torch.complex128 and tuple(ub.shape[1:]) == (3,) + self.cshape and ub.is_contiguous()
+        B = int(ub.shape[0])
+        for t, dty in ((dt, torch.float64), (Re, torch.float64), (steps, torch.int32)):
+            assert t.dtype == dty and t.numel() >= B and t.is_cuda and t.is_contiguous()
+        check(self.lib.ns3d_rk3_ssp_batch_members(self.ctx, B, nsteps, _ptr(ub), *self._f3(f), _ptr(dt), _ptr(Re),
+                                                   _ptr(steps), int(graph)), "ns3d_rk3_ssp_batch_members")
+        return ub
+
     def RK3_SSP_scalar(self, u, theta, f=None, steps=1, scheme=5):
         """RK3 of the velocity plus a WENO-advected passive scalar theta (physical float64)."""
         assert theta.dtype == torch.float64 and tuple(theta.shape) == self.rshape

@@ -80,3 +80,40 @@ def test_fd_jv_batch():
     assert float(torch.linalg.vector_norm(J2 - 2.0 * J)) < 1e-5 * nrm
     assert float(torch.linalg.vector_norm(Jc - J)) < 1e-5 * nrm
     g.close()
+
+
+@pytest.mark.parametrize("precision,graph", [("fp64", False), ("fp64", True), ("fp32", True)])
+def test_batch_members_per_member_dt_re_and_steps(precision, graph):
+    """ns3d_rk3_ssp_batch_members: each member takes its own number of the
+    launch's steps with its own dt and Re, bitwise as it would alone; an idle
+    member (0 steps) is left untouched; a second call with new parameters
+    (the same graph replayed) continues each member correctly."""
+    import torch
+    from ns3dperiodic_amd import NS3D
+    print(f"\n[case] batch members {precision} graph={graph} at {SHAPE}", flush=True)
+    f = fields.kolmogorov_forcing_sin(*SHAPE, n=4, amp=0.1)
+    fd = [torch.from_numpy(c).cuda() for c in f]
+    mem = _members(SHAPE, 4, seed=80)
+    dts = [2e-3, 1e-3, 3e-3, 2e-3]
+    res = [200.0, 80.0, 350.0, 120.0]
+    steps1 = [4, 2, 0, 3]
+    steps2 = [1, 4, 3, 0]
+    s = torch.cuda.Stream()
+    gb = NS3D(*SHAPE, Re=1.0, dt=1.0, precision=precision, stream=s)
+    ub = torch.from_numpy(np.stack([np.stack(m) for m in mem])).cuda().contiguous()
+    dt_d = torch.tensor(dts, dtype=torch.float64, device="cuda")
+    re_d = torch.tensor(res, dtype=torch.float64, device="cuda")
+    with torch.cuda.stream(s):
+        left = torch.tensor(steps1, dtype=torch.int32, device="cuda")
+        gb.RK3_SSP_batch_members(ub, fd, dt_d, re_d, left, 4, graph=graph)
+        left.copy_(torch.tensor(steps2, dtype=torch.int32))
+        gb.RK3_SSP_batch_members(ub, fd, dt_d, re_d, left, 4, graph=graph)
+    torch.cuda.synchronize()
+    for b, m in enumerate(mem):
+        g = NS3D(*SHAPE, Re=res[b], dt=dts[b], precision=precision)
+        u = [torch.from_numpy(c).cuda() for c in m]
+        g.RK3_SSP(u, fd, steps=steps1[b] + steps2[b])
+        for k in range(3):
+            assert torch.equal(ub[b, k], u[k]), (b, k)
+        g.close()
+    gb.close()

@@ -1,7 +1,8 @@
 """Continuous-batching server on the GPU.
 
 * Each request's result is bitwise the same alone and in a mix (FP32-bulk
-  and FP64), and the FP64 server reproduces one-at-a-time FP64 bitwise.
+  and FP64), batched with the other requests of its grid or on a slot of its
+  own, and the FP64 server reproduces one-at-a-time FP64 bitwise.
 * FP32-bulk results stay within 1e-5 relative L2 of FP64 over these short
   horizons (the stated tolerance of the FP32-bulk path).
 * A served request matches the CPU oracle (FP64, <= 1e-12).
@@ -53,6 +54,14 @@ def test_alone_equals_mix(precision):
     mixed = _copy(srv.run(reqs, io))
     srv.close()
     assert sorted(mixed) == [r.rid for r in reqs]
+    # the batching groups (integrate requests of one grid in one launch per pass) against
+    # every request on a slot of its own
+    srv0 = serve.Server(precision=precision, max_active=6, chunk=4, batch=False)
+    unbatched = _copy(srv0.run(reqs, io))
+    srv0.close()
+    for r in reqs:
+        for a, b in zip(unbatched[r.rid][0], mixed[r.rid][0]):
+            assert np.array_equal(a, b), f"request {r.rid} ({r.kind}, N={r.N}) differs batched vs on its own slot"
     for r in reqs:
         srv1 = serve.Server(precision=precision, max_active=1, chunk=4)
         alone = srv1.run([dataclasses.replace(r, arrival=0.0)], io)[r.rid]
