@@ -61,7 +61,7 @@ typedef struct { double x, y; } ns3d_cplx;
                                        checks for NaN in its time loop (NS3D_periodic.cu:1357-1360) */
 
 typedef struct {
-    int Nx, Ny, Nz;        /* physical grid, each in {8,16,32,48,64,96,128,192,256,512} */
+    int Nx, Ny, Nz;        /* physical grid, each in {8,16,32,48,64,80,96,128,160,192,256,320,384,512} */
     double Lx, Ly, Lz;     /* box lengths (reference: 2*pi/alpha, 2*pi, 2*pi/beta, NS3D_periodic.cu:969-979) */
     double Re;             /* Reynolds number, nu = 1/Re */
     double dt;             /* time step */

@@ -44,6 +44,32 @@ template <> struct Radix<3> {
     }
 };
 
+// 5-point DFT, W = exp(DIR 2 pi i / 5): with t1 = a1 + a4, t2 = a2 + a3,
+// t3 = a1 - a4, t4 = a2 - a3 the outputs pair up as m +- DIR i n,
+//   X1,4 = a0 + c1 t1 + c2 t2 +- DIR i (s1 t3 + s2 t4)
+//   X2,3 = a0 + c2 t1 + c1 t2 +- DIR i (s2 t3 - s1 t4)
+// (c_k = cos(2 pi k/5), s_k = sin(2 pi k/5)).
+template <> struct Radix<5> {
+    template <int DIR, typename T> __device__ __forceinline__ static void dft(cplx<T>* v) {
+        const T c1 = (T)0.30901699437494742410229341718281906;   // cos(2 pi/5)
+        const T c2 = (T)-0.80901699437494742410229341718281906;  // cos(4 pi/5)
+        const T s1 = (T)0.95105651629515357211643933337938214;   // sin(2 pi/5)
+        const T s2 = (T)0.58778525229247312916870595463907277;   // sin(4 pi/5)
+        const cplx<T> a0 = v[0];
+        const cplx<T> t1 = v[1] + v[4], t2 = v[2] + v[3];
+        const cplx<T> t3 = v[1] - v[4], t4 = v[2] - v[3];
+        const cplx<T> m1 = mk<T>(a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y);
+        const cplx<T> m2 = mk<T>(a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y);
+        const cplx<T> n1 = mul_i<DIR>(mk<T>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
+        const cplx<T> n2 = mul_i<DIR>(mk<T>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
+        v[0] = a0 + t1 + t2;
+        v[1] = m1 + n1;
+        v[4] = m1 - n1;
+        v[2] = m2 + n2;
+        v[3] = m2 - n2;
+    }
+};
+
 template <> struct Radix<4> {
     template <int DIR, typename T> __device__ __forceinline__ static void dft(cplx<T>* v) {
         cplx<T> s02 = v[0] + v[2], d02 = v[0] - v[2];
@@ -137,7 +163,9 @@ template <int A, int... R> struct SeqRev<Seq<A, R...>> {
 };
 
 // Plans: radices with at most 8 elements per thread per pass (TPP = N/8 for
-// the powers of two).  The inverse plan of the fused z-pass and the forward
+// the powers of two; the radix-3 and radix-5 passes of other lengths leave
+// some threads idle in their pass, the only pass whose butterfly count N/R
+// is not a multiple of TPP).  The inverse plan of the fused z-pass and the forward
 // plan are mirror images so the product stage needs no exchange.
 template <int N> struct Plan;
 template <> struct Plan<8> { using R = Seq<8>; static constexpr int TPP = 1; };
@@ -146,10 +174,13 @@ template <> struct Plan<24> { using R = Seq<3, 8>; static constexpr int TPP = 3;
 template <> struct Plan<32> { using R = Seq<4, 8>; static constexpr int TPP = 4; };
 template <> struct Plan<48> { using R = Seq<3, 2, 8>; static constexpr int TPP = 6; };
 template <> struct Plan<64> { using R = Seq<8, 8>; static constexpr int TPP = 8; };
+template <> struct Plan<80> { using R = Seq<5, 2, 8>; static constexpr int TPP = 10; };
 template <> struct Plan<96> { using R = Seq<3, 4, 8>; static constexpr int TPP = 12; };
 template <> struct Plan<128> { using R = Seq<2, 8, 8>; static constexpr int TPP = 16; };
+template <> struct Plan<160> { using R = Seq<5, 4, 8>; static constexpr int TPP = 20; };
 template <> struct Plan<192> { using R = Seq<3, 8, 8>; static constexpr int TPP = 24; };
 template <> struct Plan<256> { using R = Seq<4, 8, 8>; static constexpr int TPP = 32; };
+template <> struct Plan<320> { using R = Seq<5, 8, 8>; static constexpr int TPP = 40; };
 template <> struct Plan<384> { using R = Seq<3, 2, 8, 8>; static constexpr int TPP = 48; };
 template <> struct Plan<512> { using R = Seq<8, 8, 8>; static constexpr int TPP = 64; };
 
@@ -163,10 +194,13 @@ template <> struct ZPlan<24> { using R = Seq<3, 2, 4>; static constexpr int TPP 
 template <> struct ZPlan<32> { using R = Seq<2, 4, 4>; static constexpr int TPP = 8; };
 template <> struct ZPlan<48> { using R = Seq<3, 4, 4>; static constexpr int TPP = 12; };
 template <> struct ZPlan<64> { using R = Seq<4, 4, 4>; static constexpr int TPP = 16; };
+template <> struct ZPlan<80> { using R = Seq<5, 4, 4>; static constexpr int TPP = 20; };
 template <> struct ZPlan<96> { using R = Seq<3, 2, 4, 4>; static constexpr int TPP = 24; };
 template <> struct ZPlan<128> { using R = Seq<2, 4, 4, 4>; static constexpr int TPP = 32; };
+template <> struct ZPlan<160> { using R = Seq<5, 2, 4, 4>; static constexpr int TPP = 40; };
 template <> struct ZPlan<192> { using R = Seq<3, 4, 4, 4>; static constexpr int TPP = 48; };
 template <> struct ZPlan<256> { using R = Seq<4, 4, 4, 4>; static constexpr int TPP = 64; };
+template <> struct ZPlan<320> { using R = Seq<5, 4, 4, 4>; static constexpr int TPP = 80; };
 template <> struct ZPlan<384> { using R = Seq<3, 2, 4, 4, 4>; static constexpr int TPP = 96; };
 template <> struct ZPlan<512> { using R = Seq<2, 4, 4, 4, 4>; static constexpr int TPP = 128; };
 

@@ -35,18 +35,22 @@ int with_size(int n, F&& f) {
         case 32: return f(std::integral_constant<int, 32>{});
         case 48: return f(std::integral_constant<int, 48>{});
         case 64: return f(std::integral_constant<int, 64>{});
+        case 80: return f(std::integral_constant<int, 80>{});
         case 96: return f(std::integral_constant<int, 96>{});
         case 128: return f(std::integral_constant<int, 128>{});
+        case 160: return f(std::integral_constant<int, 160>{});
         case 192: return f(std::integral_constant<int, 192>{});
         case 256: return f(std::integral_constant<int, 256>{});
+        case 320: return f(std::integral_constant<int, 320>{});
+        case 384: return f(std::integral_constant<int, 384>{});
         case 512: return f(std::integral_constant<int, 512>{});
         default: return NS3D_ERR_SIZE;
     }
 }
 
 bool size_ok(int n) {
-    return n == 8 || n == 16 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 192 || n == 256 ||
-           n == 512;
+    return n == 8 || n == 16 || n == 32 || n == 48 || n == 64 || n == 80 || n == 96 || n == 128 || n == 160 ||
+           n == 192 || n == 256 || n == 320 || n == 384 || n == 512;
 }
 
 // pencils per block in the x/y passes: rows of 128 B

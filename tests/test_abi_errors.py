@@ -49,10 +49,10 @@ def test_null_arguments(lib):
 
 
 def test_status_strings_and_sizes(lib):
-    for code in (0, -1, -2, -3, -4, -5, -6):
+    for code in (0, -1, -2, -3, -4, -5, -6, -7):
         s = lib.ns3d_status_string(code).decode()
         assert s and s != lib.ns3d_status_string(12345).decode()
-    for n in (8, 16, 32, 48, 64, 96, 128, 192, 256, 512):
+    for n in (8, 16, 32, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512):
         assert lib.ns3d_supported_size(n) == 1
-    for n in (0, 4, 12, 24, 100, 384, 1024):
+    for n in (0, 4, 12, 24, 40, 100, 240, 1024):
         assert lib.ns3d_supported_size(n) == 0

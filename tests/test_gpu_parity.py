@@ -320,10 +320,12 @@ def test_shapiro_512(precision, tol):
     assert err < tol
 
 
-@pytest.mark.parametrize("shape", [(512, 16, 16), (16, 512, 16), (16, 16, 512), (16, 192, 96)],
+@pytest.mark.parametrize("shape", [(512, 16, 16), (16, 512, 16), (16, 16, 512), (16, 192, 96),
+                                   (80, 16, 16), (16, 160, 16), (16, 16, 320), (320, 16, 16), (16, 80, 160),
+                                   (384, 16, 16), (16, 16, 384)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_large_axes(shape):
-    """Largest and radix-3 plans on each axis: FFT, nonlinear term and one RK3 step."""
+    """Largest, radix-3 and radix-5 plans on each axis: FFT, nonlinear term and one RK3 step."""
     from tests.gpu_util import to_dev, to_host
     print(f"\n[case] grid {shape}", flush=True)
     g, o = _make(shape)
@@ -355,7 +357,7 @@ def test_rk3_with_passive_scalar(shape):
 
 
 @pytest.mark.parametrize("precision,tol", [("fp32", 2e-6), ("fp32s", 2e-6), ("fp32i", 2e-6)])
-@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 64, 32)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 64, 32), (80, 32, 160)], ids=lambda s: "x".join(map(str, s)))
 def test_rk3_fp32_paths(shape, precision, tol):
     """FP32 paths: five forced RK3 steps in one call against the FP64 oracle
     (the FP32-state path converts the state once on entry and once on exit)."""

@@ -13,7 +13,7 @@ import pytest
 
 from tests.conftest import rel_l2
 
-SUPPORTED = [8, 16, 32, 48, 64, 96, 128, 192, 256, 512]
+SUPPORTED = [8, 16, 32, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512]
 
 
 def brute_r2c(x):
@@ -36,7 +36,7 @@ def test_r2c_matches_brute_force(oracle_mod, shape):
     assert rel_l2(o.fft_r2c(x), brute_r2c(x)) < 1e-14
 
 
-@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 10), (32, 48, 24), (48, 16, 96), (6, 10, 14)])
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 10), (32, 48, 24), (48, 16, 96), (6, 10, 14), (80, 16, 160)])
 def test_fft_vs_numpy_and_round_trip(oracle_mod, shape):
     o = oracle_mod.Oracle(*shape)
     rng = np.random.default_rng(1)
