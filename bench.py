@@ -481,6 +481,44 @@ def copy_peak_gbs(reps=5, kinds=("copy", "scale")):
             "kernel": best, "GBps_by_kernel": {k: 2 * n * 8 / (v * 1e-3) / 1e9 for k, v in rates.items()}}
 
 
+# the access pattern of each pass's memory traffic, as measured by pattern_ceiling:
+# (pattern, fields read, fields written) of its nearest copy kernel
+PASS_PATTERN = {"k_x_inv_grad": ("x_pencil", 3, 6), "k_y_inv_grad": ("y_pencil", 6, 6),
+                "k_z_product": ("z_rows_truncated", 9, 3), "k_axis_c2c": ("y_pencil", 3, 3),
+                "k_x_fwd_epi": ("x_pencil", 6, 3)}
+
+
+def pattern_ceilings(N, precisions=("fp64", "fp32"), reps=20):
+    """Copy-only kernels with the passes' tiles (ns3dperiodic_amd/pattern_ceiling,
+    built with the library): the bandwidth each access pattern allows on this GPU.
+    Returns the list of measured patterns, or None if the program is not built."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ns3dperiodic_amd", "pattern_ceiling")
+    if not os.path.exists(exe):
+        print(f"[bench] {exe} is not built (python -m ns3dperiodic_amd.build): no pattern ceilings",
+              file=sys.stderr, flush=True)
+        return None
+    out = []
+    for p in precisions:
+        r = subprocess.run([exe, str(N), p, str(reps)], capture_output=True, text=True, timeout=120)
+        if r.returncode != 0:
+            raise RuntimeError(f"pattern_ceiling {N} {p} failed: {r.stderr.strip()}")
+        out += [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
+    return out
+
+
+def ceiling_of(ceilings, kernel, precision):
+    """The measured ceiling of a pass's access pattern (bulk precision of its scratch)."""
+    if not ceilings or kernel not in PASS_PATTERN:
+        return None
+    pat, nin, nout = PASS_PATTERN[kernel]
+    bulk = "fp64" if precision == "fp64" else "fp32"
+    for c in ceilings:
+        if (c["pattern"], c["read_fields"], c["write_fields"], c["precision"]) == (pat, nin, nout, bulk):
+            return c
+    return None
+
+
 def host_cpu():
     """CPU model, logical CPUs and the CPUs this process may run on."""
     model = None
@@ -879,6 +917,21 @@ def main():
                 "traffic_source": "profiles/rk3_256_passes_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, "
                                   "scaled by a plain-copy calibration)" if traffic else None,
                 "alg_bytes_per_launch": dom["alg_bytes"], "avg_launch_ms": dom["avg_ms"]}
+    ceilings = None if args.no_extra else pattern_ceilings(N)
+    if ceilings:
+        print(f"[bench] pattern ceilings: { {(c['pattern'], c['precision'], c['read_fields'], c['write_fields']): round(c['GBps']) for c in ceilings} }",
+              file=sys.stderr, flush=True)
+        ce = ceiling_of(ceilings, dom["kernel"], args.precision)
+        if ce:
+            actual = traffic / (dom["avg_ms"] * 1e-3) / 1e9 if traffic else None
+            roofline["pattern_ceiling"] = {
+                "pattern": f"{ce['pattern']} {ce['read_fields']}->{ce['write_fields']} ({ce['precision']})",
+                "GBps": ce["GBps"], "actual_traffic_GBps": actual,
+                "frac": actual / ce["GBps"] if actual else None,
+                "source": "ns3dperiodic_amd/pattern_ceiling: copy-only kernel with the pass's tiles, same run"}
+        for q in prim["passes"]:
+            c = ceiling_of(ceilings, q["kernel"], args.precision)
+            q["pattern_ceiling_GBps"] = c["GBps"] if c else None
 
     cpu = parity = None
     if not args.no_cpu:
@@ -920,6 +973,7 @@ def main():
         "finite": prim["finite"],
         "serving": serving,
         "fp32_accuracy": accuracy,
+        "pattern_ceilings": ceilings,
     }
     if extra is not None:
         cp = extra["copy_peak"]["GBps"]

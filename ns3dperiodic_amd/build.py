@@ -48,7 +48,23 @@ def build(verbose=True, extra_flags=()):
     os.replace(tmp, LIB)
     if verbose:
         print(f"[build] {LIB} ({time.time() - t0:.1f} s)", flush=True)
+    build_pattern_ceiling(verbose)
     return LIB
+
+
+# The access-pattern bandwidth benchmark (a standalone program bench.py runs).
+PATTERN_BIN = os.path.join(HERE, "pattern_ceiling")
+
+
+def build_pattern_ceiling(verbose=True):
+    flags = [f for f in HIPCC_FLAGS if f not in ("-shared", "-fPIC")]
+    tmp = PATTERN_BIN + ".tmp"
+    cmd = [hipcc(), *flags, os.path.join(CSRC, "pattern_ceiling.hip"), "-o", tmp]
+    if verbose:
+        print("[build]", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, PATTERN_BIN)
+    return PATTERN_BIN
 
 
 if __name__ == "__main__":
