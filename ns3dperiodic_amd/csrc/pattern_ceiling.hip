@@ -2,7 +2,7 @@
 // allows on this GPU, measured with kernels that only move the bytes.
 //
 // The five passes read and write spectral fields (N x N x (N/2+1) complex
-// modes, rows padded to Mi modes: 136 FP64 / 144 FP32 at N = 256) along three
+// modes, rows padded to Mi modes: 136 at N = 256) along three
 // kinds of pencils:
 //   x   the x-inverse and x-forward passes: P consecutive z-modes l of one
 //       (k) row per block, every x-plane j (planes N*Mi modes apart);
@@ -62,7 +62,7 @@ constexpr int lmax_of(int mz) {
 template <typename T, int NX>
 struct Geo {
     static constexpr int N = NX, Mz = NX / 2 + 1;
-    static constexpr int pad = sizeof(T) == 8 ? 8 : 16;
+    static constexpr int pad = 8;
     static constexpr int Mi = (Mz + pad - 1) / pad * pad;  // rows padded as the passes' scratch
     static constexpr int lmax = lmax_of(Mz);
     static constexpr long nc = (long)NX * NX * Mi;          // modes per field

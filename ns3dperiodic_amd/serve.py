@@ -127,14 +127,14 @@ def field_bytes(N, precision):
     """HBM held by an integrate request on an N^3 grid, in the library's layout:
     3 state fields and the forcing x-component (FP64, reference layout); 9 FP64
     RK stage fields (rows padded to 8 modes); 15 scratch fields in the bulk
-    precision (rows padded to 8 modes FP64, 16 FP32); for "fp32s" the FP32
-    state and stages of the RK3 loop (9 fields, rows padded to 16) replace the
-    FP64 stage fields."""
+    precision (rows padded to 8 modes); for "fp32s" the FP32 state and
+    stages of the RK3 loop (9 fields, rows padded to 8) replace the FP64
+    stage fields."""
     nc = N * N * (N // 2 + 1)
-    bulk, pad = (16, 8) if precision == "fp64" else (8, 16)
+    bulk, pad = (16, 8) if precision == "fp64" else (8, 8)
     b = 16 * 4 * nc + bulk * 15 * _rows(N, pad)
     # the FP32-state loop keeps its stages in FP32 (no FP64 stage buffer)
-    b += 8 * 9 * _rows(N, 16) if precision == "fp32s" else 16 * 9 * _rows(N, 8)
+    b += 8 * 9 * _rows(N, 8) if precision == "fp32s" else 16 * 9 * _rows(N, 8)
     return b
 
 
@@ -147,7 +147,7 @@ def request_bytes(req, precision):
     if req.kind == "return_map":
         b += nc * 16 * 3              # the state saved before each step (bisection restart)
     if req.kind == "jv":
-        bulk, pad = (16, 8) if precision == "fp64" else (8, 16)
+        bulk, pad = (16, 8) if precision == "fp64" else (8, 8)
         b += 16 * 3 * nc + 16 * 6 * _rows(req.N, 8) + bulk * 15 * _rows(req.N, pad)
     return b
 
@@ -320,7 +320,7 @@ def group_member_bytes(N, precision):
     3 FP64 state fields (reference layout), 6 FP64 RK stage fields (rows padded
     to 8 modes) and 15 scratch fields in the bulk precision (padded rows)."""
     nc = N * N * (N // 2 + 1)
-    bulk, pad = (16, 8) if precision == "fp64" else (8, 16)
+    bulk, pad = (16, 8) if precision == "fp64" else (8, 8)
     return 16 * 3 * nc + 16 * 6 * _rows(N, 8) + bulk * 15 * _rows(N, pad)
 
 

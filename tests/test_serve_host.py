@@ -58,14 +58,14 @@ def test_request_bytes():
     r = serve.Request(0, "integrate", 64, 100.0, 1.0, 0, 0.0)
     j = serve.Request(1, "jv", 64, 100.0, 1.0, 0, 0.0)
     nc = 64 * 64 * 33
-    n8, n16 = 64 * 64 * 40, 64 * 64 * 48       # rows of Mz = 33 padded to 8 / 16 modes
+    n8 = 64 * 64 * 40                          # rows of Mz = 33 padded to 8 modes (FP64 and FP32 fields)
     # 4 resident FP64 fields (state + forcing x), 9 FP64 stage fields, 15 scratch fields
     assert serve.request_bytes(r, "fp64") == 16 * 4 * nc + 16 * 9 * n8 + 16 * 15 * n8
-    assert serve.request_bytes(r, "fp32") == 16 * 4 * nc + 16 * 9 * n8 + 8 * 15 * n16
+    assert serve.request_bytes(r, "fp32") == 16 * 4 * nc + 16 * 9 * n8 + 8 * 15 * n8
     # FP32 state loop: 9 FP32 fields replace the FP64 stages
-    assert serve.request_bytes(r, "fp32s") == 16 * 4 * nc + 8 * 9 * n16 + 8 * 15 * n16
+    assert serve.request_bytes(r, "fp32s") == 16 * 4 * nc + 8 * 9 * n8 + 8 * 15 * n8
     # J.v: perturbation (3 FP64), tangent stage (6 FP64 padded), 15 more scratch fields
-    assert serve.request_bytes(j, "fp32") == serve.request_bytes(r, "fp32") + 16 * 3 * nc + 16 * 6 * n8 + 8 * 15 * n16
+    assert serve.request_bytes(j, "fp32") == serve.request_bytes(r, "fp32") + 16 * 3 * nc + 16 * 6 * n8 + 8 * 15 * n8
 
 
 def test_ledger_limits():
