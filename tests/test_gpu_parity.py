@@ -340,6 +340,34 @@ def test_large_axes(shape):
     assert rel_l2(to_host(ud), o.rk3(u, zero)) < TOL
 
 
+@pytest.mark.parametrize("shape", [(80, 16, 32), (16, 32, 80), (32, 160, 16), (16, 16, 320), (48, 16, 96)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_bilinear_tangent_and_jv_on_radix5_and_radix3_axes(shape):
+    """The bilinear term B(u,v), the tangent-map family and the reduced J.v
+    (their own z-pass and epilogue kernels) on radix-5 and radix-3 axes
+    against the oracle."""
+    import torch
+    from tests.gpu_util import to_dev, to_host
+    print(f"\n[case] bilinear / tangent / J.v grid {shape}", flush=True)
+    g, o = _make(shape)
+    u = _field(shape, seed=41)
+    v = _field(shape, seed=42, energy=1e-3)
+    f = fields.kolmogorov_forcing_sin(*shape, n=1)
+    assert rel_l2(to_host(g.calculate_convolution_2p3_UV(to_dev(u), to_dev(v))), o.bilinear(u, v)) < TOL
+    assert rel_l2(to_host(g.RK3_SSP_UV_RHS(to_dev(u), to_dev(v))), o.rk3_uv_rhs(u, v)) < TOL
+    assert rel_l2(to_host(g.single_time_step_iUV_RHS(to_dev(u), to_dev(v))), o.iuv_rhs(u, v)) < TOL
+    ud, vd = to_dev(u), to_dev(v)
+    g.RK3_SSP_UV(ud, vd, to_dev(f))
+    uo, vo = o.rk3_uv(u, v, f)
+    assert rel_l2(to_host(ud), uo) < TOL
+    assert rel_l2(to_host(vd), vo) < TOL
+    n = g.vector_length_reduced()
+    vec = np.random.default_rng(43).standard_normal(n)
+    g.set_base_state(to_dev(u))
+    out = to_host([g.jv_reduced(torch.from_numpy(vec).cuda())])[0]
+    assert rel_l2(out, o.jv_reduced(u, vec)) < TOL
+
+
 @pytest.mark.parametrize("shape", [(32, 32, 32), (32, 16, 48)], ids=lambda s: "x".join(map(str, s)))
 def test_rk3_with_passive_scalar(shape):
     """Velocity RK3 plus a WENO5-advected scalar (the decaying-turbulence passive scalar) against the oracle."""
