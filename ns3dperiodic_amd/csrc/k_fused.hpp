@@ -57,6 +57,12 @@ template <typename T, int N> struct PtrArr { T* p[N]; };
 #ifndef NS3D_REV_XFWD
 #define NS3D_REV_XFWD 0
 #endif
+// Stage a pass's twiddle table after its first global loads are issued (1) or
+// before them (0): the table's own round trip and block barrier then overlap
+// the HBM latency of the pencil loads.
+#ifndef NS3D_TW_LATE
+#define NS3D_TW_LATE 1
+#endif
 template <int REV>
 __device__ __forceinline__ int tile_order(int bid, int nb) {
     return REV ? nb - 1 - bid : bid;
@@ -118,7 +124,7 @@ k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc
     // a column (k,l) whose every x mode is masked stays unwritten: the y pass skips it
     const bool keep = lv && column_kept(tab, k, l);
     if (!__syncthreads_or(keep)) return;
-    stage_twiddles<TB, NX>(tw, twg);
+    if (!NS3D_TW_LATE) stage_twiddles<TB, NX>(tw, twg);
     const long S = (long)Ny * Mw;        // state: row stride Mw (Mz: reference layout)
     const long base = (long)k * Mw + l;
     const long So = (long)Ny * Ms;       // scratch: rows padded to Ms
@@ -129,20 +135,32 @@ k_x_inv_grad(PtrArr<const cplx<TS>, 3 * NF> w, cplx<TB>* __restrict__ A, long nc
     const cplx<TS>* __restrict__ src2 = ADD ? w2.p[f % 3] : nullptr;
     A += (long)blockIdx.z * bsc;
     cplx<TB> v[F::E], d[OS ? 1 : F::E];
+    // the loads first (raw), then the twiddle table, then the arithmetic
+    cplx<TS> raw[F::E], raw2[ADD != 0 ? F::E : 1];
 #pragma unroll
     for (int q = 0; q < F::B0; q++) {
 #pragma unroll
         for (int r = 0; r < F::R0; r++) {
             const int j = F::in_index(t, q, r);
-            cplx<double> s = mk<double>(0.0, 0.0);
-            double kx = 0.0;
-            if (F::in_active(t, q)) {
-                kx = tab.kx[j];
-                if (keep && mask_keep(tab, j, k, l)) {
-                    s = cvt<double>(src[base + j * S]);
-                    if constexpr (ADD != 0) s = s + cvt<double>(src2[base + j * S]);
-                }
+            const int i = q * F::R0 + r;
+            raw[i] = mk<TS>(0, 0);
+            if constexpr (ADD != 0) raw2[i] = mk<TS>(0, 0);
+            if (F::in_active(t, q) && keep && mask_keep(tab, j, k, l)) {
+                raw[i] = src[base + j * S];
+                if constexpr (ADD != 0) raw2[i] = src2[base + j * S];
             }
+        }
+    }
+    if (NS3D_TW_LATE) stage_twiddles<TB, NX>(tw, twg);
+#pragma unroll
+    for (int q = 0; q < F::B0; q++) {
+#pragma unroll
+        for (int r = 0; r < F::R0; r++) {
+            const int j = F::in_index(t, q, r);
+            const int i = q * F::R0 + r;
+            cplx<double> s = cvt<double>(raw[i]);
+            if constexpr (ADD != 0) s = s + cvt<double>(raw2[i]);
+            const double kx = F::in_active(t, q) ? tab.kx[j] : 0.0;
             const cplx<TB> dv = mk<TB>((TB)(-s.y * kx), (TB)(s.x * kx));
             if constexpr (OS) {
                 v[q * F::R0 + r] = which ? dv : cvt<TB>(s);
@@ -193,7 +211,7 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
     B += (long)blockIdx.z * bsc;
     const int l = lt * P + p;
     const bool lv = l < Mz && l <= tab.lmax;
-    stage_twiddles<TB, NY>(tw, twg);
+    if (!NS3D_TW_LATE) stage_twiddles<TB, NY>(tw, twg);
     const long S = Ms;
     const long base = (long)j * Ny * Ms + l;
     const int f = OS ? (int)blockIdx.y / 3 : (int)blockIdx.y;
@@ -227,11 +245,18 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
             for (int r = 0; r < F::R0; r++) {
                 const int kk = F::in_index(t, q, r);
                 cplx<TB> s = mk<TB>(0, 0);
-                TB ky = 0;
-                if (F::in_active(t, q)) {
-                    ky = (TB)tab.ky[kk];
-                    if (lv && column_kept(tab, kk, l)) s = src[base + kk * S];
-                }
+                if (F::in_active(t, q) && lv && column_kept(tab, kk, l)) s = src[base + kk * S];
+                v[q * F::R0 + r] = s;
+            }
+        }
+        if (NS3D_TW_LATE) stage_twiddles<TB, NY>(tw, twg);
+#pragma unroll
+        for (int q = 0; q < F::B0; q++) {
+#pragma unroll
+            for (int r = 0; r < F::R0; r++) {
+                const int kk = F::in_index(t, q, r);
+                const cplx<TB> s = v[q * F::R0 + r];
+                const TB ky = F::in_active(t, q) ? (TB)tab.ky[kk] : (TB)0;
                 const cplx<TB> dv = mk<TB>(-s.y * ky, s.x * ky);
                 if constexpr (OS) {
                     v[q * F::R0 + r] = which ? dv : s;
@@ -277,6 +302,8 @@ k_y_inv_grad(const cplx<TB>* __restrict__ A, cplx<TB>* __restrict__ B, long nc, 
                 }
             }
         }
+        // with OS these blocks skip the velocity branch, where the table is staged otherwise
+        if (NS3D_TW_LATE && OS) stage_twiddles<TB, NY>(tw, twg);
         F::run(v, lds, tw, t, p);
         cplx<TB>* __restrict__ ow = B + (long)(s_ * 9 + 3 + b) * nc;
         if (lv) {
@@ -529,7 +556,8 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
     const int bid0 = xcd_tile();
     if (bid0 >= nb || member_idle(mm)) return;
     const int bid = tile_order<NS3D_REV_Z>(bid0, nb);
-    stage_twiddles<TB, NZ>(tw, twg);
+    // NF = 1 stages the table after its first loads are issued
+    if (NF != 1 || !NS3D_TW_LATE) stage_twiddles<TB, NZ>(tw, twg);
     const int tid = threadIdx.x, t = tid % TPP, p = tid / TPP;
     const long pen = (long)bid * P + p;
     const bool valid = pen < npen;
@@ -549,6 +577,7 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
         Z::kz_regs(kz, tab, t);
         Z::fetch_pair(r1, r2, fld(0), fld(1), valid, tab, t);
         Z::fetch_pair(r3, r4, fld(2), fld(3), valid, tab, t);
+        if (NS3D_TW_LATE) stage_twiddles<TB, NZ>(tw, twg);
         auto run_two = [&](TB* x1, TB* x2, bool d1, bool d2, TB* x3, TB* x4, bool d3, bool d4, const cplx<TB>* n1,
                            const cplx<TB>* n2, const cplx<TB>* n3, const cplx<TB>* n4) {
             cplx<TB> va[Z::E], vb[Z::E];
@@ -597,6 +626,7 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
         cplx<TB> r1[Z::NIN], r2[Z::NIN];
         Z::kz_regs(kz, tab, t);
         Z::fetch_pair(r1, r2, fld(0), fld(1), valid, tab, t);
+        if (NS3D_TW_LATE) stage_twiddles<TB, NZ>(tw, twg);
         auto run_pair = [&](TB* x1, TB* x2, bool d1, bool d2, const cplx<TB>* n1, const cplx<TB>* n2) {
             cplx<TB> v[Z::E];
             Z::finish_pair(v, r1, d1, r2, d2, kz, t);
@@ -699,7 +729,8 @@ k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, i
     const int bid0 = xcd_tile();
     if (bid0 >= nb || member_idle(mm)) return;
     const int bid = tile_order<NS3D_REV_YFWD>(bid0, nb);
-    stage_twiddles<TB, NA>(tw, twg);
+    if (!NS3D_TW_LATE) stage_twiddles<TB, NA>(tw, twg);
+    bool staged = !NS3D_TW_LATE;  // block-uniform
     const int tid = threadIdx.x, p = tid % P, t = tid / P;
     const int o = bid / nlt, lt = bid % nlt;
     const int l = lt * P + p;
@@ -721,6 +752,10 @@ k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, i
                 if (F::in_active(t, q) && lv) s = src[base + F::in_index(t, q, r) * S];
                 v[q * F::R0 + r] = s;
             }
+        }
+        if (!staged) {
+            stage_twiddles<TB, NA>(tw, twg);
+            staged = true;
         }
         __syncthreads();  // in-place use: every pencil element is read before any is written
         F::run(v, lds, tw, t, p);
@@ -925,7 +960,8 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int
     const int bid0 = xcd_tile();
     if (bid0 >= nb || member_idle(mm)) return;
     const int bid = tile_order<NS3D_REV_XFWD>(bid0, nb);
-    stage_twiddles<TB, NX>(tw, twg);
+    constexpr bool tw_late = NS3D_TW_LATE && NS3D_HOIST;
+    if (!tw_late) stage_twiddles<TB, NX>(tw, twg);
     const int tid = threadIdx.x, p = tid % P, t = tid / P;
     const int k = bid / nlt, lt = bid % nlt;
     const int l = lt * P + p;
@@ -965,6 +1001,7 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int
         }
         if (!NS3D_HOIST) F::run(qv[b], lds, tw, t, p);
     }
+    if (tw_late) stage_twiddles<TB, NX>(tw, twg);
     if constexpr (ILP) {
         F::run2(qv[0], qv[1], lds, lds + NX * P, tw, t, p);
         F::run(qv[2], lds, tw, t, p);
