@@ -57,9 +57,11 @@ template <typename T, int N> struct PtrArr { T* p[N]; };
 #ifndef NS3D_REV_XFWD
 #define NS3D_REV_XFWD 0
 #endif
-// Stage a pass's twiddle table after its first global loads are issued (1) or
-// before them (0): the table's own round trip and block barrier then overlap
-// the HBM latency of the pencil loads.
+// The x- and y-inverse passes stage their twiddle table after their first
+// global loads are issued (1) or before them (0): the table's own round trip
+// and block barrier then overlap the HBM latency of the pencil loads.  The
+// other passes stage it first: issuing their loads first made the 512^3 z pass
+// 3-4% (FP64) and the FP32 y-forward pass 5% slower, with no gain at 256^3.
 #ifndef NS3D_TW_LATE
 #define NS3D_TW_LATE 1
 #endif
@@ -556,8 +558,7 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
     const int bid0 = xcd_tile();
     if (bid0 >= nb || member_idle(mm)) return;
     const int bid = tile_order<NS3D_REV_Z>(bid0, nb);
-    // NF = 1 stages the table after its first loads are issued
-    if (NF != 1 || !NS3D_TW_LATE) stage_twiddles<TB, NZ>(tw, twg);
+    stage_twiddles<TB, NZ>(tw, twg);
     const int tid = threadIdx.x, t = tid % TPP, p = tid / TPP;
     const long pen = (long)bid * P + p;
     const bool valid = pen < npen;
@@ -577,7 +578,6 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
         Z::kz_regs(kz, tab, t);
         Z::fetch_pair(r1, r2, fld(0), fld(1), valid, tab, t);
         Z::fetch_pair(r3, r4, fld(2), fld(3), valid, tab, t);
-        if (NS3D_TW_LATE) stage_twiddles<TB, NZ>(tw, twg);
         auto run_two = [&](TB* x1, TB* x2, bool d1, bool d2, TB* x3, TB* x4, bool d3, bool d4, const cplx<TB>* n1,
                            const cplx<TB>* n2, const cplx<TB>* n3, const cplx<TB>* n4) {
             cplx<TB> va[Z::E], vb[Z::E];
@@ -626,7 +626,6 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
         cplx<TB> r1[Z::NIN], r2[Z::NIN];
         Z::kz_regs(kz, tab, t);
         Z::fetch_pair(r1, r2, fld(0), fld(1), valid, tab, t);
-        if (NS3D_TW_LATE) stage_twiddles<TB, NZ>(tw, twg);
         auto run_pair = [&](TB* x1, TB* x2, bool d1, bool d2, const cplx<TB>* n1, const cplx<TB>* n2) {
             cplx<TB> v[Z::E];
             Z::finish_pair(v, r1, d1, r2, d2, kz, t);
@@ -729,8 +728,7 @@ k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, i
     const int bid0 = xcd_tile();
     if (bid0 >= nb || member_idle(mm)) return;
     const int bid = tile_order<NS3D_REV_YFWD>(bid0, nb);
-    if (!NS3D_TW_LATE) stage_twiddles<TB, NA>(tw, twg);
-    bool staged = !NS3D_TW_LATE;  // block-uniform
+    stage_twiddles<TB, NA>(tw, twg);
     const int tid = threadIdx.x, p = tid % P, t = tid / P;
     const int o = bid / nlt, lt = bid % nlt;
     const int l = lt * P + p;
@@ -752,10 +750,6 @@ k_axis_c2c(const cplx<TB>* in, cplx<TB>* out, int nf, long nc, int Ny, int Mz, i
                 if (F::in_active(t, q) && lv) s = src[base + F::in_index(t, q, r) * S];
                 v[q * F::R0 + r] = s;
             }
-        }
-        if (!staged) {
-            stage_twiddles<TB, NA>(tw, twg);
-            staged = true;
         }
         __syncthreads();  // in-place use: every pencil element is read before any is written
         F::run(v, lds, tw, t, p);
@@ -960,8 +954,7 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int
     const int bid0 = xcd_tile();
     if (bid0 >= nb || member_idle(mm)) return;
     const int bid = tile_order<NS3D_REV_XFWD>(bid0, nb);
-    constexpr bool tw_late = NS3D_TW_LATE && NS3D_HOIST;
-    if (!tw_late) stage_twiddles<TB, NX>(tw, twg);
+    stage_twiddles<TB, NX>(tw, twg);
     const int tid = threadIdx.x, p = tid % P, t = tid / P;
     const int k = bid / nlt, lt = bid % nlt;
     const int l = lt * P + p;
@@ -1001,7 +994,6 @@ k_x_fwd_epi(const cplx<TB>* __restrict__ D, long nc, int Ny, int Mz, int Ms, int
         }
         if (!NS3D_HOIST) F::run(qv[b], lds, tw, t, p);
     }
-    if (tw_late) stage_twiddles<TB, NX>(tw, twg);
     if constexpr (ILP) {
         F::run2(qv[0], qv[1], lds, lds + NX * P, tw, t, p);
         F::run(qv[2], lds, tw, t, p);
