@@ -1,30 +1,35 @@
 // tuned.hpp -- written by ns3dperiodic_amd/autotune.py (select); do not edit by hand.
 // Timed on AMD Radeon Graphics (gfx950): RK3 steps at 256^3 (power-of-two parameters) and 192^3 (NS3D_R3_*); a value replaces the baseline only when its step is faster by more than 0.5%.
 #pragma once
-// NS3D_EXINV64 = 4 (kept; fp64 step at 256^3 4.4189 ms summed)
-// NS3D_EYINV64 = 4 (kept; fp64 step at 256^3 4.4189 ms summed)
-// NS3D_EXFWD64 = 8 (kept; fp64 step at 256^3 4.4189 ms summed)
-// NS3D_PXINV64 = 8 (kept; fp64 step at 256^3 4.4189 ms summed)
-// NS3D_PYINV64 = 8 (kept; fp64 step at 256^3 4.4189 ms summed)
-// NS3D_PZT64 = 128 (kept; fp64 step at 256^3 4.4189 ms summed)
-// NS3D_ZSWZ = 16 (kept; fp64 step at 256^3 4.4189 ms summed)
-// NS3D_ZILP = 1 (kept; fp64 step at 256^3 4.4189 ms summed)
-// NS3D_XFWD_ILP = 2 (kept; fp64, fp32, fp32s step at 256^3 10.0354 ms summed)
-// NS3D_EXINV32 = 8 (kept; fp32, fp32s step at 256^3 5.6165 ms summed)
-// NS3D_EYINV32 = 4 (kept; fp32, fp32s step at 256^3 5.6165 ms summed)
-// NS3D_PXINV32 = 8 (kept; fp32, fp32s step at 256^3 5.6165 ms summed)
-// NS3D_PYINV32 = 8 (kept; fp32, fp32s step at 256^3 5.6165 ms summed)
-// NS3D_PYFWD32 = 16 (kept; fp32, fp32s step at 256^3 5.6165 ms summed)
-// NS3D_PXFWD32 = 8 (kept; fp32, fp32s step at 256^3 5.6165 ms summed)
-// NS3D_PZT32 = 128 (kept; fp32, fp32s step at 256^3 5.6165 ms summed)
-// NS3D_ZPLAN32 = 8 (kept; fp32, fp32s step at 256^3 5.6165 ms summed)
-// NS3D_ZSWZ32 = 16 (kept; fp32, fp32s step at 256^3 5.6165 ms summed)
-// NS3D_R3_PXINV64 = 4 (kept; fp64 step at 192^3 2.3815 ms summed)
-// NS3D_R3_PYINV64 = 4 (kept; fp64 step at 192^3 2.3815 ms summed)
-// NS3D_R3_PZT64 = 256 (kept; fp64 step at 192^3 2.3815 ms summed)
-// NS3D_R3_ZSWZ = 0 (kept; fp64 step at 192^3 2.3815 ms summed)
-// NS3D_R3_PXFWD32 = 16 (kept; fp32, fp32s step at 192^3 2.7264 ms summed)
-// NS3D_R3_PYINV32 = 4 (kept; fp32, fp32s step at 192^3 2.7264 ms summed)
+// NS3D_EXINV64 = 4 (kept; fp64 step at 256^3 4.3406 ms summed)
+// NS3D_EYINV64 = 4 (kept; fp64 step at 256^3 4.3406 ms summed)
+// NS3D_EXFWD64 = 8 (kept; fp64 step at 256^3 4.3406 ms summed)
+// NS3D_PXINV64 = 8 (kept; fp64 step at 256^3 4.3406 ms summed)
+// NS3D_PYINV64 = 8 (kept; fp64 step at 256^3 4.3406 ms summed)
+// NS3D_PZT64 = 128 (kept; fp64 step at 256^3 4.3406 ms summed)
+// NS3D_ZSWZ = 16 (kept; fp64 step at 256^3 4.3406 ms summed)
+// NS3D_ZILP = 1 (kept; fp64 step at 256^3 4.3406 ms summed)
+// NS3D_XFWD_ILP = 2 (kept; fp64, fp32, fp32s, fp32i step at 256^3 12.4749 ms summed)
+// NS3D_EXINV32 = 8 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
+// NS3D_EYINV32 = 4 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
+// NS3D_PXINV32 = 8 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
+// NS3D_PYINV32 = 8 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
+// NS3D_PYFWD32 = 16 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
+// NS3D_PXFWD32 = 8 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
+// NS3D_PZT32 = 128 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
+// NS3D_ZPLAN32 = 8 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
+// NS3D_ZSWZ32 = 16 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
+// NS3D_EXFWD32 = 4 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
+// NS3D_ZILP32 = 0 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
+// NS3D_ZWAVE = 1 (kept; fp64, fp32, fp32s, fp32i step at 256^3 12.4749 ms summed)
+// NS3D_WAVES_Z32 = 2 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
+// NS3D_WAVES_XFWD32 = 2 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
+// NS3D_R3_PXINV64 = 4 (kept; fp64 step at 192^3 2.3676 ms summed)
+// NS3D_R3_PYINV64 = 4 (kept; fp64 step at 192^3 2.3676 ms summed)
+// NS3D_R3_PZT64 = 256 (kept; fp64 step at 192^3 2.3676 ms summed)
+// NS3D_R3_ZSWZ = 0 (kept; fp64 step at 192^3 2.3676 ms summed)
+// NS3D_R3_PXFWD32 = 16 (kept; fp32, fp32s, fp32i step at 192^3 4.1344 ms summed)
+// NS3D_R3_PYINV32 = 4 (kept; fp32, fp32s, fp32i step at 192^3 4.1344 ms summed)
 #define NS3D_EXINV64 4
 #define NS3D_EYINV64 4
 #define NS3D_EXFWD64 8
@@ -43,6 +48,11 @@
 #define NS3D_PZT32 128
 #define NS3D_ZPLAN32 8
 #define NS3D_ZSWZ32 16
+#define NS3D_EXFWD32 4
+#define NS3D_ZILP32 0
+#define NS3D_ZWAVE 1
+#define NS3D_WAVES_Z32 2
+#define NS3D_WAVES_XFWD32 2
 #define NS3D_R3_PXINV64 4
 #define NS3D_R3_PYINV64 4
 #define NS3D_R3_PZT64 256
