@@ -425,6 +425,25 @@ struct ZPencil {
             }
         }
     }
+    // fetch_pair through buffer resources b1, b2 over the block's valid pencils:
+    // prow = this pencil's first element in them (p * row stride); masked and
+    // truncated elements and padding pencils fall outside the range and read 0.
+    __device__ __forceinline__ static void fetch_pair_buf(cplx<TB>* r1, cplx<TB>* r2, __amdgpu_buffer_rsrc_t b1,
+                                                          __amdgpu_buffer_rsrc_t b2, int prow, const SpecTables& tab,
+                                                          int t) {
+#pragma unroll
+        for (int q = 0; q < FI::B0; q++) {
+#pragma unroll
+            for (int r = 0; r < FI::R0; r++) {
+                const int n = FI::in_index(t, q, r);
+                const int sl = n <= NZ / 2 ? n : NZ - n;
+                const bool on = FI::in_active(t, q) && (!TRUNC || sl <= tab.lmax);
+                const int o = on ? (prow + sl) * (int)sizeof(cplx<TB>) : BUF_OOB;
+                r1[q * FI::R0 + r] = buf_load<TB>(b1, o);
+                r2[q * FI::R0 + r] = buf_load<TB>(b2, o);
+            }
+        }
+    }
     __device__ __forceinline__ static void finish_pair(cplx<TB>* v, const cplx<TB>* r1, bool d1, const cplx<TB>* r2,
                                                        bool d2, const TB* kz, int t) {
 #pragma unroll
@@ -566,6 +585,23 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
     Bf += (long)blockIdx.z * bsc;
     C += (long)blockIdx.z * bsc;
     auto fld = [&](int f) { return Bf + (long)f * nc + off; };
+#ifndef NS3D_ZBUF
+#define NS3D_ZBUF 1
+#endif
+    // the first-pass inputs of fields fa and fb (the pipelined paths): with
+    // NS3D_ZBUF through buffer resources over the block's valid pencils
+    // (branch-free, no zero fill), else by predicated global loads
+    const long pen0 = (long)bid * P;
+    const int rbytes = (int)((npen - pen0 < P ? npen - pen0 : P) * Mz * (long)sizeof(cplx<TB>));
+    auto fetch = [&](cplx<TB>* ra, cplx<TB>* rb, int fa, int fb) {
+        if constexpr (NS3D_ZBUF != 0) {
+            const cplx<TB>* b0 = Bf + pen0 * Mz;
+            Z::fetch_pair_buf(ra, rb, buf_rsrc(b0 + (long)fa * nc, rbytes), buf_rsrc(b0 + (long)fb * nc, rbytes),
+                              p * Mz, tab, t);
+        } else {
+            Z::fetch_pair(ra, rb, fld(fa), fld(fb), valid, tab, t);
+        }
+    };
     // field ids: s*9 + {0..2: w_b, 3..5: dx w_b, 6..8: dy w_b}
     TB Q[3][NPT];
     if constexpr (ILP) {
@@ -576,16 +612,17 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
         TB kz[Z::NIN];
         cplx<TB> r1[Z::NIN], r2[Z::NIN], r3[Z::NIN], r4[Z::NIN];
         Z::kz_regs(kz, tab, t);
-        Z::fetch_pair(r1, r2, fld(0), fld(1), valid, tab, t);
-        Z::fetch_pair(r3, r4, fld(2), fld(3), valid, tab, t);
-        auto run_two = [&](TB* x1, TB* x2, bool d1, bool d2, TB* x3, TB* x4, bool d3, bool d4, const cplx<TB>* n1,
-                           const cplx<TB>* n2, const cplx<TB>* n3, const cplx<TB>* n4) {
+        fetch(r1, r2, 0, 1);
+        fetch(r3, r4, 2, 3);
+        // n1..n4: the fields of the next two pairs (-1: none)
+        auto run_two = [&](TB* x1, TB* x2, bool d1, bool d2, TB* x3, TB* x4, bool d3, bool d4, int n1, int n2, int n3,
+                           int n4) {
             cplx<TB> va[Z::E], vb[Z::E];
             Z::finish_pair(va, r1, d1, r2, d2, kz, t);
             Z::finish_pair(vb, r3, d3, r4, d4, kz, t);
-            if (n1) {
-                Z::fetch_pair(r1, r2, n1, n2, valid, tab, t);
-                Z::fetch_pair(r3, r4, n3, n4, valid, tab, t);
+            if (n1 >= 0) {
+                fetch(r1, r2, n1, n2);
+                fetch(r3, r4, n3, n4);
             }
             Z::FI::run2(va, vb, lds, lb, tw, t, p);
 #pragma unroll
@@ -597,7 +634,7 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
             }
         };
         // (w_x, w_y), (w_z, dx w_x)
-        run_two(w[0], w[1], false, false, w[2], g0, false, false, fld(6), fld(0), fld(4), fld(7));
+        run_two(w[0], w[1], false, false, w[2], g0, false, false, 6, 0, 4, 7);
         if (uphys && valid) {
 #pragma unroll
             for (int e = 0; e < NPT; e++) {
@@ -608,11 +645,11 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
             }
         }
         // (dy w_x, dz w_x), (dx w_y, dy w_y)
-        run_two(g1, g2, false, true, g3, g4, false, false, fld(1), fld(5), fld(8), fld(2));
+        run_two(g1, g2, false, true, g3, g4, false, false, 1, 5, 8, 2);
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[0][e] = w[0][e] * g0[e] + w[1][e] * g1[e] + w[2][e] * g2[e];
         // (dz w_y, dx w_z), (dy w_z, dz w_z)
-        run_two(g0, Q[2], true, false, g1, g2, false, true, nullptr, nullptr, nullptr, nullptr);
+        run_two(g0, Q[2], true, false, g1, g2, false, true, -1, -1, -1, -1);
 #pragma unroll
         for (int e = 0; e < NPT; e++) {
             Q[1][e] = w[0][e] * g3[e] + w[1][e] * g4[e] + w[2][e] * g0[e];
@@ -625,11 +662,11 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
         TB kz[Z::NIN];
         cplx<TB> r1[Z::NIN], r2[Z::NIN];
         Z::kz_regs(kz, tab, t);
-        Z::fetch_pair(r1, r2, fld(0), fld(1), valid, tab, t);
-        auto run_pair = [&](TB* x1, TB* x2, bool d1, bool d2, const cplx<TB>* n1, const cplx<TB>* n2) {
+        fetch(r1, r2, 0, 1);
+        auto run_pair = [&](TB* x1, TB* x2, bool d1, bool d2, int n1, int n2) {   // n1, n2: next pair (-1: none)
             cplx<TB> v[Z::E];
             Z::finish_pair(v, r1, d1, r2, d2, kz, t);
-            if (n1) Z::fetch_pair(r1, r2, n1, n2, valid, tab, t);
+            if (n1 >= 0) fetch(r1, r2, n1, n2);
             Z::FI::run(v, lds, tw, t, p);
 #pragma unroll
             for (int e = 0; e < NPT; e++) {
@@ -637,8 +674,8 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
                 x2[e] = v[e].y * scale;
             }
         };
-        run_pair(w[0], w[1], false, false, fld(2), fld(3));   // w_x, w_y
-        run_pair(w[2], keep, false, false, fld(6), fld(0));   // w_z, dx w_x
+        run_pair(w[0], w[1], false, false, 2, 3);   // w_x, w_y
+        run_pair(w[2], keep, false, false, 6, 0);   // w_z, dx w_x
         if (uphys && valid) {
 #pragma unroll
             for (int e = 0; e < NPT; e++) {
@@ -648,14 +685,14 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
                 uphys[2 * nr + g] = w[2][e];
             }
         }
-        run_pair(a, b, false, true, fld(4), fld(7));          // dy w_x, dz w_x
+        run_pair(a, b, false, true, 4, 7);          // dy w_x, dz w_x
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[0][e] = w[0][e] * keep[e] + w[1][e] * a[e] + w[2][e] * b[e];
-        run_pair(keep, a, false, false, fld(1), fld(5));      // dx w_y, dy w_y
-        run_pair(b, Q[2], true, false, fld(8), fld(2));       // dz w_y, dx w_z
+        run_pair(keep, a, false, false, 1, 5);      // dx w_y, dy w_y
+        run_pair(b, Q[2], true, false, 8, 2);       // dz w_y, dx w_z
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[1][e] = w[0][e] * keep[e] + w[1][e] * a[e] + w[2][e] * b[e];
-        run_pair(a, b, false, true, nullptr, nullptr);        // dy w_z, dz w_z
+        run_pair(a, b, false, true, -1, -1);        // dy w_z, dz w_z
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[2][e] = w[0][e] * Q[2][e] + w[1][e] * a[e] + w[2][e] * b[e];
     } else {

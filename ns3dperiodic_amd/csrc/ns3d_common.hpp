@@ -26,6 +26,31 @@ template <int SGN, typename T> __device__ __forceinline__ cplx<T> mul_i(cplx<T> 
 }
 template <typename TO, typename TI> __device__ __forceinline__ cplx<TO> cvt(cplx<TI> a) { return mk<TO>((TO)a.x, (TO)a.y); }
 
+// Buffer-resource (MUBUF) loads.  A raw buffer resource covers `bytes` bytes
+// from `base`; a load whose byte offset is at or beyond that range returns
+// zeros without a memory access.  An element that must read as zero (masked,
+// truncated, a padding pencil) is loaded at offset BUF_OOB instead of being
+// zero-filled and skipped by a branch, and offsets are 32-bit.  Word 3 of the
+// resource (0x00020000) sets a 32-bit data format: with a zero format the
+// range-checked loads of this GPU family return zeros.
+constexpr int BUF_OOB = (int)0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
+}
+template <typename T> __device__ __forceinline__ cplx<T> buf_load(__amdgpu_buffer_rsrc_t r, int off);
+template <> __device__ __forceinline__ cplx<double> buf_load<double>(__amdgpu_buffer_rsrc_t r, int off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    cplx<double> a;
+    __builtin_memcpy(&a, &v, sizeof(a));
+    return a;
+}
+template <> __device__ __forceinline__ cplx<float> buf_load<float>(__amdgpu_buffer_rsrc_t r, int off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    cplx<float> a;
+    __builtin_memcpy(&a, &v, sizeof(a));
+    return a;
+}
+
 // Per-context spectral constants.  Per-axis tables are built on the host with
 // the same floating-point expressions as the reference's host builders
 // (Source/math_support.cu:374-519) so that on-the-fly recomposition on the
