@@ -57,6 +57,8 @@ SPACE = {
     "NS3D_EXFWD32": (4, [4, 8], ["fp32", "fp32s", "fp32i"], "pow2"),
     "NS3D_ZILP32": (0, [0, 1], ["fp32", "fp32s", "fp32i"], "pow2"),
     "NS3D_ZWAVE": (1, [0, 1], ["fp64", "fp32", "fp32s", "fp32i"], "pow2"),
+    "NS3D_ZTWREG": (1, [0, 1], ["fp64"], "pow2"),
+    "NS3D_ZTWREG32": (0, [0, 1], ["fp32", "fp32s", "fp32i"], "pow2"),
     "NS3D_WAVES_Z32": (2, [2, 3], ["fp32", "fp32s", "fp32i"], "pow2"),
     "NS3D_WAVES_XFWD32": (2, [2, 3, 4], ["fp32", "fp32s", "fp32i"], "pow2"),
     "NS3D_R3_PXINV64": (4, [4, 8], ["fp64"], "r3"),

@@ -281,10 +281,48 @@ struct PencilFFT {
     __device__ __forceinline__ static bool out_active(int t, int q) { return (N / RL) % TPP == 0 || t + q * TPP < N / RL; }
     __device__ __forceinline__ static int out_index(int t, int q, int r) { return t + q * TPP + r * (N / RL); }
 
-    // tw[m] = exp(-2*pi*i*m/N); DIR=+1 uses the conjugate.
-    // Butterflies of pass I (twiddles, then the radix-R DFTs) on this thread's values.
+    // Register twiddles: a thread that runs many transforms of one plan (the
+    // z pass) reads its twiddles of every pass from the table once (load_tw)
+    // instead of in every transform.  Slot tw_off(I) + q*(R-1) + r-1 holds
+    // tw[r*jm*N/(Ns*R)] of pass I, as stored in the table.
+    static constexpr int tw_count(int i) {
+        return Info::prod_before(i) > 1 ? ((N / Info::rad[i] + TPP - 1) / TPP) * (Info::rad[i] - 1) : 0;
+    }
+    static constexpr int tw_off(int i) {
+        int o = 0;
+        for (int m = 0; m < i; m++) o += tw_count(m);
+        return o;
+    }
+    struct TwReg {
+        cplx<T> w[tw_off(NP) > 0 ? tw_off(NP) : 1];
+    };
     template <int I>
-    __device__ __forceinline__ static void butterflies(cplx<T>* v, const cplx<T>* __restrict__ tw, int t) {
+    __device__ __forceinline__ static void load_tw_pass(TwReg& tr, const cplx<T>* __restrict__ tw, int t) {
+        constexpr int R = Info::rad[I];
+        constexpr int Ns = Info::prod_before(I);
+        constexpr int NB = N / R;
+        constexpr int B = (NB + TPP - 1) / TPP;
+        if constexpr (Ns > 1) {
+#pragma unroll
+            for (int q = 0; q < B; q++) {
+                const int jj = t + q * TPP;
+                const int jm = (NB % TPP == 0 || jj < NB) ? jj % Ns : 0;
+#pragma unroll
+                for (int r = 1; r < R; r++) tr.w[tw_off(I) + q * (R - 1) + r - 1] = tw[r * jm * (N / (Ns * R))];
+            }
+        }
+        if constexpr (I + 1 < NP) load_tw_pass<I + 1>(tr, tw, t);
+    }
+    __device__ __forceinline__ static void load_tw(TwReg& tr, const cplx<T>* __restrict__ tw, int t) {
+        load_tw_pass<0>(tr, tw, t);
+    }
+
+    // tw[m] = exp(-2*pi*i*m/N); DIR=+1 uses the conjugate.
+    // Butterflies of pass I (twiddles, then the radix-R DFTs) on this thread's
+    // values; twiddles from the table tw, or with tr from registers.
+    template <int I>
+    __device__ __forceinline__ static void butterflies(cplx<T>* v, const cplx<T>* __restrict__ tw, int t,
+                                                       const TwReg* tr = nullptr) {
         constexpr int R = Info::rad[I];
         constexpr int Ns = Info::prod_before(I);
         constexpr int NB = N / R;
@@ -297,7 +335,7 @@ struct PencilFFT {
                     const int jm = jj % Ns;
 #pragma unroll
                     for (int r = 1; r < R; r++) {
-                        cplx<T> w = tw[r * jm * (N / (Ns * R))];
+                        cplx<T> w = tr ? tr->w[tw_off(I) + q * (R - 1) + r - 1] : tw[r * jm * (N / (Ns * R))];
                         if (DIR > 0) w.y = -w.y;
                         v[q * R + r] = cmul(v[q * R + r], w);
                     }
@@ -340,23 +378,24 @@ struct PencilFFT {
     }
 
     template <int I>
-    __device__ __forceinline__ static void pass(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t, int p) {
-        butterflies<I>(v, tw, t);
+    __device__ __forceinline__ static void pass(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t, int p,
+                                                const TwReg* tr) {
+        butterflies<I>(v, tw, t, tr);
         if constexpr (I + 1 < NP) {
             exch_write<I>(v, lds, t, p);
             exch_sync<WAVE>();
             exch_read<I>(v, lds, t, p);
             exch_sync<WAVE>();
-            pass<I + 1>(v, lds, tw, t, p);
+            pass<I + 1>(v, lds, tw, t, p, tr);
         }
     }
     // Two independent transforms in step: one barrier per exchange for both,
     // and each one's LDS round trip overlaps the other's butterflies.
     template <int I>
     __device__ __forceinline__ static void pass2(cplx<T>* va, cplx<T>* vb, cplx<T>* la, cplx<T>* lb,
-                                                 const cplx<T>* __restrict__ tw, int t, int p) {
-        butterflies<I>(va, tw, t);
-        butterflies<I>(vb, tw, t);
+                                                 const cplx<T>* __restrict__ tw, int t, int p, const TwReg* tr) {
+        butterflies<I>(va, tw, t, tr);
+        butterflies<I>(vb, tw, t, tr);
         if constexpr (I + 1 < NP) {
             exch_write<I>(va, la, t, p);
             exch_write<I>(vb, lb, t, p);
@@ -364,19 +403,27 @@ struct PencilFFT {
             exch_read<I>(va, la, t, p);
             exch_read<I>(vb, lb, t, p);
             exch_sync<WAVE>();
-            pass2<I + 1>(va, vb, la, lb, tw, t, p);
+            pass2<I + 1>(va, vb, la, lb, tw, t, p, tr);
         }
     }
 
     // Full transform: v holds the first-pass input pattern on entry and the
     // last-pass output pattern on exit.  All threads of the block must call.
     __device__ __forceinline__ static void run(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t, int p) {
-        pass<0>(v, lds, tw, t, p);
+        pass<0>(v, lds, tw, t, p, nullptr);
     }
     // Two transforms at once; la and lb are disjoint exchange buffers of the block's layout.
     __device__ __forceinline__ static void run2(cplx<T>* va, cplx<T>* vb, cplx<T>* la, cplx<T>* lb,
                                                 const cplx<T>* __restrict__ tw, int t, int p) {
-        pass2<0>(va, vb, la, lb, tw, t, p);
+        pass2<0>(va, vb, la, lb, tw, t, p, nullptr);
+    }
+    // run / run2 with the twiddles in registers (load_tw)
+    __device__ __forceinline__ static void run_r(cplx<T>* v, cplx<T>* lds, const TwReg& tr, int t, int p) {
+        pass<0>(v, lds, nullptr, t, p, &tr);
+    }
+    __device__ __forceinline__ static void run2_r(cplx<T>* va, cplx<T>* vb, cplx<T>* la, cplx<T>* lb, const TwReg& tr,
+                                                  int t, int p) {
+        pass2<0>(va, vb, la, lb, nullptr, t, p, &tr);
     }
 };
 

@@ -572,6 +572,17 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
     // lengths (256^3: z pass 0.458 -> 0.406 ms).  Measured no gain in FP32, and
     // register spills at the radix-3 lengths.
     constexpr bool ILP = NF == 1 && (NZ & (NZ - 1)) == 0 && (sizeof(TB) == 8 ? NS3D_ZILP != 0 : NS3D_ZILP32 != 0);
+#ifndef NS3D_ZTWREG
+#define NS3D_ZTWREG 1
+#endif
+#ifndef NS3D_ZTWREG32
+#define NS3D_ZTWREG32 0
+#endif
+    // The inverse transforms' twiddles in registers, read from the table once
+    // (NZ >= 32; below, the extra registers cost a wave per SIMD).  FP64 at
+    // 256^3: z pass 0.375 -> 0.352 ms, at 512^3 3.68 -> 3.54 ms (+42 and +24
+    // VGPRs, no spills); FP32 measured 1% slower and stays on the LDS table.
+    constexpr bool TWR = NF == 1 && NZ >= 32 && (sizeof(TB) == 8 ? NS3D_ZTWREG != 0 : NS3D_ZTWREG32 != 0);
     __shared__ cplx<TB> lds[(ILP ? 2 : 1) * P * (NZ + Z::PAD)];
     __shared__ cplx<TB> tw[NZ];
     const int bid0 = xcd_tile();
@@ -602,6 +613,8 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
             Z::fetch_pair(ra, rb, fld(fa), fld(fb), valid, tab, t);
         }
     };
+    typename Z::FI::TwReg twi;
+    if constexpr (TWR) Z::FI::load_tw(twi, tw, t);
     // field ids: s*9 + {0..2: w_b, 3..5: dx w_b, 6..8: dy w_b}
     TB Q[3][NPT];
     if constexpr (ILP) {
@@ -624,7 +637,10 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
                 fetch(r1, r2, n1, n2);
                 fetch(r3, r4, n3, n4);
             }
-            Z::FI::run2(va, vb, lds, lb, tw, t, p);
+            if constexpr (TWR)
+                Z::FI::run2_r(va, vb, lds, lb, twi, t, p);
+            else
+                Z::FI::run2(va, vb, lds, lb, tw, t, p);
 #pragma unroll
             for (int e = 0; e < NPT; e++) {
                 x1[e] = va[e].x * scale;
@@ -667,7 +683,10 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
             cplx<TB> v[Z::E];
             Z::finish_pair(v, r1, d1, r2, d2, kz, t);
             if (n1 >= 0) fetch(r1, r2, n1, n2);
-            Z::FI::run(v, lds, tw, t, p);
+            if constexpr (TWR)
+                Z::FI::run_r(v, lds, twi, t, p);
+            else
+                Z::FI::run(v, lds, tw, t, p);
 #pragma unroll
             for (int e = 0; e < NPT; e++) {
                 x1[e] = v[e].x * scale;
