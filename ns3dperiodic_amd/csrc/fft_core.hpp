@@ -264,6 +264,13 @@ template <bool WAVE> __device__ __forceinline__ void exch_sync() {
     }
 }
 
+// Per-thread register twiddles of a plan (PencilFFT::load_tw); plans with the
+// same radix sequence share the slot layout, and then one set serves both
+// transform directions.
+template <typename T, int K> struct TwRegs {
+    cplx<T> w[K];
+};
+
 template <typename T, int N, int TPP, int DIR, typename S, typename Addr, bool WAVE = false>
 struct PencilFFT {
     using Info = SeqInfo<S>;
@@ -293,9 +300,7 @@ struct PencilFFT {
         for (int m = 0; m < i; m++) o += tw_count(m);
         return o;
     }
-    struct TwReg {
-        cplx<T> w[tw_off(NP) > 0 ? tw_off(NP) : 1];
-    };
+    using TwReg = TwRegs<T, (tw_off(NP) > 0 ? tw_off(NP) : 1)>;
     template <int I>
     __device__ __forceinline__ static void load_tw_pass(TwReg& tr, const cplx<T>* __restrict__ tw, int t) {
         constexpr int R = Info::rad[I];

@@ -484,16 +484,26 @@ struct ZPencil {
     // Forward-transform (q1 + i q2) and q3 at once (two exchange buffers) and
     // store the three half spectra: the results of r2c_pair(q1, q2) and
     // r2c_pair(q3, null), with one barrier per exchange for both.
+    // With twr (the inverse plan's register twiddles; FI and FF must share
+    // their radix sequence, as the palindromic power-of-two z plans do) the
+    // forward transforms read their twiddles from registers too.
+    static constexpr bool SAME_PLAN = std::is_same<typename ZPlanT<TB, NZ>::R,
+                                                   typename SeqRev<typename ZPlanT<TB, NZ>::R>::type>::value;
+    template <bool TWR>
     __device__ __forceinline__ static void r2c_two(const TB* q1, const TB* q2, const TB* q3, cplx<TB>* __restrict__ o1,
                                                    cplx<TB>* __restrict__ o2, cplx<TB>* __restrict__ o3, bool valid,
-                                                   cplx<TB>* la, cplx<TB>* lb, const cplx<TB>* tw, int t, int p) {
+                                                   cplx<TB>* la, cplx<TB>* lb, const cplx<TB>* tw, int t, int p,
+                                                   const typename FI::TwReg& twr) {
         cplx<TB> va[E], vb[E];
 #pragma unroll
         for (int e = 0; e < NPT; e++) {
             va[e] = mk<TB>(q1[e], q2[e]);
             vb[e] = mk<TB>(q3[e], (TB)0);
         }
-        FF::run2(va, vb, la, lb, tw, t, p);
+        if constexpr (TWR && SAME_PLAN)
+            FF::run2_r(va, vb, la, lb, twr, t, p);
+        else
+            FF::run2(va, vb, la, lb, tw, t, p);
 #pragma unroll
         for (int q = 0; q < FF::BL; q++) {
             if (!FF::out_active(t, q)) continue;
@@ -583,6 +593,9 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
     // 256^3: z pass 0.375 -> 0.352 ms, at 512^3 3.68 -> 3.54 ms (+42 and +24
     // VGPRs, no spills); FP32 measured 1% slower and stays on the LDS table.
     constexpr bool TWR = NF == 1 && NZ >= 32 && (sizeof(TB) == 8 ? NS3D_ZTWREG != 0 : NS3D_ZTWREG32 != 0);
+#ifndef NS3D_ZTWFWD
+#define NS3D_ZTWFWD 1
+#endif
     __shared__ cplx<TB> lds[(ILP ? 2 : 1) * P * (NZ + Z::PAD)];
     __shared__ cplx<TB> tw[NZ];
     const int bid0 = xcd_tile();
@@ -763,7 +776,8 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
     cplx<TB>* o1 = C + nc + off;
     cplx<TB>* o2 = C + 2 * nc + off;
     if constexpr (ILP) {
-        Z::r2c_two(Q[0], Q[1], Q[2], o0, o1, o2, valid, lds, lds + P * (NZ + Z::PAD), tw, t, p);
+        Z::template r2c_two<TWR && NS3D_ZTWFWD != 0>(Q[0], Q[1], Q[2], o0, o1, o2, valid, lds, lds + P * (NZ + Z::PAD),
+                                                     tw, t, p, twi);
     } else {
         Z::r2c_pair(Q[0], Q[1], o0, o1, valid, lds, tw, t, p);
         Z::r2c_pair(Q[2], nullptr, o2, nullptr, valid, lds, tw, t, p);
