@@ -467,6 +467,48 @@ struct ZPencil {
         }
     }
 
+    // load_pair through buffer resources over the block's valid pencils (as
+    // fetch_pair_buf): out-of-range elements read as zero.
+    __device__ __forceinline__ static void load_pair_buf(cplx<TB>* v, __amdgpu_buffer_rsrc_t b1, bool d1,
+                                                         __amdgpu_buffer_rsrc_t b2, bool d2, int prow,
+                                                         const SpecTables& tab, int t) {
+#pragma unroll
+        for (int q = 0; q < FI::B0; q++) {
+#pragma unroll
+            for (int r = 0; r < FI::R0; r++) {
+                const int n = FI::in_index(t, q, r);
+                const int sl = n <= NZ / 2 ? n : NZ - n;
+                const bool on = FI::in_active(t, q) && (!TRUNC || sl <= tab.lmax);
+                const int o = on ? (prow + sl) * (int)sizeof(cplx<TB>) : BUF_OOB;
+                cplx<TB> a = buf_load<TB>(b1, o), b = buf_load<TB>(b2, o);
+                const TB kz = FI::in_active(t, q) ? (TB)tab.kz[sl] : (TB)0;
+                if (d1) a = mk<TB>(-a.y * kz, a.x * kz);
+                if (d2) b = mk<TB>(-b.y * kz, b.x * kz);
+                if (n == 0 || n == NZ / 2) {
+                    a.y = 0;
+                    b.y = 0;
+                } else if (n > NZ / 2) {
+                    a.y = -a.y;
+                    b.y = -b.y;
+                }
+                v[q * FI::R0 + r] = mk<TB>(a.x - b.y, a.y + b.x);
+            }
+        }
+    }
+    __device__ __forceinline__ static void c2r_pair_buf(TB* x1, TB* x2, __amdgpu_buffer_rsrc_t b1, bool d1,
+                                                        __amdgpu_buffer_rsrc_t b2, bool d2, int prow,
+                                                        const SpecTables& tab, cplx<TB>* lds, const cplx<TB>* tw,
+                                                        int t, int p, TB scale) {
+        cplx<TB> v[E];
+        load_pair_buf(v, b1, d1, b2, d2, prow, tab, t);
+        FI::run(v, lds, tw, t, p);
+#pragma unroll
+        for (int e = 0; e < NPT; e++) {
+            x1[e] = v[e].x * scale;
+            x2[e] = v[e].y * scale;
+        }
+    }
+
     // Inverse-transform a pair; x1/x2 receive the scaled real fields.
     __device__ __forceinline__ static void c2r_pair(TB* x1, TB* x2, const cplx<TB>* s1, bool d1, const cplx<TB>* s2, bool d2,
                                                     bool valid, const SpecTables& tab, cplx<TB>* lds,
@@ -558,13 +600,19 @@ struct ZPencil {
 };
 
 template <typename TB, int NZ, int P, int NF>
-// The bilinear z pass (24 real fields per point) spills at 2 waves/SIMD; at 1
-// the compiler keeps the overflow in AGPRs instead of scratch memory (256^3
-// tangent-map step: FP64 89.7 -> 92.2, FP32 127.5 -> 141.4 steps/s).
+// The bilinear z pass (24 real fields per point): with its inputs through
+// buffer loads the FP32 pass fits 2 waves/SIMD in 240 VGPRs whatever the
+// bound; in FP64 2 waves/SIMD cost 92 B/lane of scratch spills at 256^3 and
+// still beat 1 wave with the overflow in AGPRs (256^3 tangent-map step
+// 10.13 -> 9.46 ms, 128^3 1.35 -> 1.31 ms).
 #ifndef NS3D_WAVES_Z2
-#define NS3D_WAVES_Z2 1
+#define NS3D_WAVES_Z2 2
 #endif
-__global__ void __attribute__((amdgpu_waves_per_eu(NF == 1 ? (sizeof(TB) == 4 ? NS3D_WAVES_Z32 : NS3D_WAVES_Z) : NS3D_WAVES_Z2, 8)))
+#ifndef NS3D_WAVES_Z2_32
+#define NS3D_WAVES_Z2_32 1
+#endif
+__global__ void __attribute__((amdgpu_waves_per_eu(NF == 1 ? (sizeof(TB) == 4 ? NS3D_WAVES_Z32 : NS3D_WAVES_Z)
+                                                          : (sizeof(TB) == 4 ? NS3D_WAVES_Z2_32 : NS3D_WAVES_Z2), 8)))
 __launch_bounds__((P * ZPlanT<TB, NZ>::TPP))
 k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, long npen, int Mz, int nb,
             SpecTables tab, const cplx<TB>* __restrict__ twg, TB scale, TB* __restrict__ uphys, long nr, long bsc = 0,
@@ -617,14 +665,19 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
     // (branch-free, no zero fill), else by predicated global loads
     const long pen0 = (long)bid * P;
     const int rbytes = (int)((npen - pen0 < P ? npen - pen0 : P) * Mz * (long)sizeof(cplx<TB>));
+    auto rsrc = [&](int f) { return buf_rsrc(Bf + pen0 * Mz + (long)f * nc, rbytes); };
     auto fetch = [&](cplx<TB>* ra, cplx<TB>* rb, int fa, int fb) {
-        if constexpr (NS3D_ZBUF != 0) {
-            const cplx<TB>* b0 = Bf + pen0 * Mz;
-            Z::fetch_pair_buf(ra, rb, buf_rsrc(b0 + (long)fa * nc, rbytes), buf_rsrc(b0 + (long)fb * nc, rbytes),
-                              p * Mz, tab, t);
-        } else {
+        if constexpr (NS3D_ZBUF != 0)
+            Z::fetch_pair_buf(ra, rb, rsrc(fa), rsrc(fb), p * Mz, tab, t);
+        else
             Z::fetch_pair(ra, rb, fld(fa), fld(fb), valid, tab, t);
-        }
+    };
+    // the bilinear path's paired inverse transform of fields fa, fb (d: times i kz)
+    auto c2r = [&](TB* x1, TB* x2, int fa, bool d1, int fb, bool d2) {
+        if constexpr (NS3D_ZBUF != 0)
+            Z::c2r_pair_buf(x1, x2, rsrc(fa), d1, rsrc(fb), d2, p * Mz, tab, lds, tw, t, p, scale);
+        else
+            Z::c2r_pair(x1, x2, fld(fa), d1, fld(fb), d2, valid, tab, lds, tw, t, p, scale);
     };
     typename Z::FI::TwReg twi;
     if constexpr (TWR) Z::FI::load_tw(twi, tw, t);
@@ -735,40 +788,40 @@ k_z_product(const cplx<TB>* __restrict__ Bf, cplx<TB>* __restrict__ C, long nc, 
         // the larger of its two fields.
         // ids: u_b = b, dx u_b = 3+b, dy u_b = 6+b; v_b = 9+b, dx v_b = 12+b, dy v_b = 15+b
         TB u[3][NPT], vv[3][NPT], a[NPT], b[NPT];
-        Z::c2r_pair(u[0], u[1], fld(0), false, fld(1), false, valid, tab, lds, tw, t, p, scale);
-        Z::c2r_pair(vv[0], vv[1], fld(9), false, fld(10), false, valid, tab, lds, tw, t, p, scale);
-        Z::c2r_pair(u[2], a, fld(2), false, fld(3), false, valid, tab, lds, tw, t, p, scale);      // u_z, dx u_x
-        Z::c2r_pair(vv[2], b, fld(11), false, fld(12), false, valid, tab, lds, tw, t, p, scale);   // v_z, dx v_x
+        c2r(u[0], u[1], 0, false, 1, false);
+        c2r(vv[0], vv[1], 9, false, 10, false);
+        c2r(u[2], a, 2, false, 3, false);      // u_z, dx u_x
+        c2r(vv[2], b, 11, false, 12, false);   // v_z, dx v_x
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[0][e] = vv[0][e] * a[e] + u[0][e] * b[e];
-        Z::c2r_pair(a, b, fld(6), false, fld(0), true, valid, tab, lds, tw, t, p, scale);          // dy u_x, dz u_x
+        c2r(a, b, 6, false, 0, true);          // dy u_x, dz u_x
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[0][e] += vv[1][e] * a[e] + vv[2][e] * b[e];
-        Z::c2r_pair(a, b, fld(15), false, fld(9), true, valid, tab, lds, tw, t, p, scale);         // dy v_x, dz v_x
+        c2r(a, b, 15, false, 9, true);         // dy v_x, dz v_x
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[0][e] += u[1][e] * a[e] + u[2][e] * b[e];
-        Z::c2r_pair(a, b, fld(4), false, fld(7), false, valid, tab, lds, tw, t, p, scale);         // dx u_y, dy u_y
+        c2r(a, b, 4, false, 7, false);         // dx u_y, dy u_y
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[1][e] = vv[0][e] * a[e] + vv[1][e] * b[e];
-        Z::c2r_pair(a, b, fld(1), true, fld(5), false, valid, tab, lds, tw, t, p, scale);          // dz u_y, dx u_z
+        c2r(a, b, 1, true, 5, false);          // dz u_y, dx u_z
 #pragma unroll
         for (int e = 0; e < NPT; e++) {
             Q[1][e] += vv[2][e] * a[e];
             Q[2][e] = vv[0][e] * b[e];
         }
-        Z::c2r_pair(a, b, fld(13), false, fld(16), false, valid, tab, lds, tw, t, p, scale);       // dx v_y, dy v_y
+        c2r(a, b, 13, false, 16, false);       // dx v_y, dy v_y
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[1][e] += u[0][e] * a[e] + u[1][e] * b[e];
-        Z::c2r_pair(a, b, fld(10), true, fld(14), false, valid, tab, lds, tw, t, p, scale);        // dz v_y, dx v_z
+        c2r(a, b, 10, true, 14, false);        // dz v_y, dx v_z
 #pragma unroll
         for (int e = 0; e < NPT; e++) {
             Q[1][e] += u[2][e] * a[e];
             Q[2][e] += u[0][e] * b[e];
         }
-        Z::c2r_pair(a, b, fld(8), false, fld(2), true, valid, tab, lds, tw, t, p, scale);          // dy u_z, dz u_z
+        c2r(a, b, 8, false, 2, true);          // dy u_z, dz u_z
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[2][e] += vv[1][e] * a[e] + vv[2][e] * b[e];
-        Z::c2r_pair(a, b, fld(17), false, fld(11), true, valid, tab, lds, tw, t, p, scale);        // dy v_z, dz v_z
+        c2r(a, b, 17, false, 11, true);        // dy v_z, dz v_z
 #pragma unroll
         for (int e = 0; e < NPT; e++) Q[2][e] += u[1][e] * a[e] + u[2][e] * b[e];
     }
