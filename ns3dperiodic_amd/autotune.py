@@ -42,7 +42,7 @@ SPACE = {
     "NS3D_PXINV64": (8, [4, 8], ["fp64"], "pow2"),
     "NS3D_PYINV64": (8, [4, 8], ["fp64"], "pow2"),
     "NS3D_PZT64": (128, [64, 128, 256], ["fp64"], "pow2"),
-    "NS3D_ZSWZ": (16, [0, 16], ["fp64"], "pow2"),
+    "NS3D_ZSWZ": (8, [0, 8, 16], ["fp64"], "pow2"),
     "NS3D_ZILP": (1, [0, 1], ["fp64"], "pow2"),
     "NS3D_XFWD_ILP": (2, [0, 1, 2], ["fp64", "fp32", "fp32s", "fp32i"], "pow2"),
     "NS3D_EXINV32": (8, [4, 8], ["fp32", "fp32s", "fp32i"], "pow2"),

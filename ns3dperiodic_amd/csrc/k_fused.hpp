@@ -328,7 +328,7 @@ struct ZPencil {
     static constexpr int TPP = ZPlanT<TB, NZ>::TPP;
     static constexpr int PAD = (sizeof(TB) == 8) ? 1 : 2;
 #ifndef NS3D_ZSWZ
-#define NS3D_ZSWZ 16
+#define NS3D_ZSWZ 8
 #endif
 #ifndef NS3D_ZSWZ32
 #define NS3D_ZSWZ32 16

@@ -1,4 +1,4 @@
-// tuned.hpp -- written by ns3dperiodic_amd/autotune.py (select); do not edit by hand.
+// tuned.hpp -- written by ns3dperiodic_amd/autotune.py (select), then NS3D_ZSWZ set from per-pass timings (its note).
 // Timed on AMD Radeon Graphics (gfx950): RK3 steps at 256^3 (power-of-two parameters) and 192^3 (NS3D_R3_*); a value replaces the baseline only when its step is faster by more than 0.5%.
 #pragma once
 // NS3D_EXINV64 = 4 (kept; fp64 step at 256^3 4.3406 ms summed)
@@ -7,7 +7,7 @@
 // NS3D_PXINV64 = 8 (kept; fp64 step at 256^3 4.3406 ms summed)
 // NS3D_PYINV64 = 8 (kept; fp64 step at 256^3 4.3406 ms summed)
 // NS3D_PZT64 = 128 (kept; fp64 step at 256^3 4.3406 ms summed)
-// NS3D_ZSWZ = 16 (kept; fp64 step at 256^3 4.3406 ms summed)
+// NS3D_ZSWZ = 8 (per-pass A/B/A: FP64 z pass -1.6% / -1.3% / -2.9% at 128^3 / 256^3 / 512^3 against 16, bitwise-identical; below the 0.5% step margin)
 // NS3D_ZILP = 1 (kept; fp64 step at 256^3 4.3406 ms summed)
 // NS3D_XFWD_ILP = 2 (kept; fp64, fp32, fp32s, fp32i step at 256^3 12.4749 ms summed)
 // NS3D_EXINV32 = 8 (kept; fp32, fp32s, fp32i step at 256^3 8.1343 ms summed)
@@ -36,7 +36,7 @@
 #define NS3D_PXINV64 8
 #define NS3D_PYINV64 8
 #define NS3D_PZT64 128
-#define NS3D_ZSWZ 16
+#define NS3D_ZSWZ 8
 #define NS3D_ZILP 1
 #define NS3D_XFWD_ILP 2
 #define NS3D_EXINV32 8
